@@ -1,0 +1,193 @@
+// MI355X (gfx950) implementation of the 4d N=4 SYM RHMC hot path:
+// twisted fermion operator, multi-shift CG, fermion and gauge forces.
+//
+// One shared library per gauge group U(NCOL), NCOL in {2, 3, 4}:
+//   libsusy_hip_U<NCOL>.so.  Define NCOL to the library's value before
+//   including this header; the Twist_Fermion layout depends on it.
+//
+// Three layers of entry points:
+//  1. Drop-in symbols with the reference's exact C signatures
+//     (4d_Q16/susy/susy_includes.h:73-81).  They read the caller's globals
+//     through the pointers registered with susy_bind_*() (the reference keeps
+//     them as globals in susy/lattice.h:71-185) and move Twist_Fermion arrays
+//     between host and device on every call.
+//  2. Binding calls a reference driver makes once (and after every link
+//     update) so that the drop-in symbols see its state.
+//  3. A device-resident API used by this project's own RHMC driver, tests and
+//     benchmark: fields stay in HBM between calls.
+//
+// Errors: entry points returning int return 0 on success and a negative
+// SUSY_E* code on failure; susy_last_error() gives a message.  A bad sign in
+// fermion_op is fatal, as in the reference (utilities.c:789-792).
+#ifndef SUSY_HIP_H
+#define SUSY_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef NCOL
+#error "define NCOL (2, 3 or 4) to match the library before including susy_hip.h"
+#endif
+
+// ---------------------------------------------------------------------
+// Types, layout-compatible with the reference (include/susy.h:29-44,
+// susy/lattice.h:18-22) at PRECISION=2: Real = double.
+typedef double Real;
+typedef struct { double real, imag; } susy_complex;
+typedef struct { susy_complex e[NCOL][NCOL]; } susy_matrix;
+typedef struct {
+  susy_matrix Fsite;
+  susy_matrix Flink[5];
+  susy_matrix Fplaq[10];
+} Twist_Fermion;
+
+enum {
+  SUSY_OK = 0,
+  SUSY_E_ARG = -1,      // invalid argument (sizes, sign, unsupported option)
+  SUSY_E_STATE = -2,    // no lattice bound / not initialised
+  SUSY_E_HIP = -3,      // HIP runtime error
+  SUSY_E_NCOL = -4,     // library built for another NCOL
+  SUSY_E_NOMEM = -5,    // device allocation failed
+};
+
+// ---------------------------------------------------------------------
+// 1. Drop-in replacements (susy_includes.h:73-81)
+// dest = D src (sign = +1) or D^dagger src (sign = -1)   [utilities.c:765]
+void fermion_op(Twist_Fermion *src, Twist_Fermion *dest, int sign);
+// dest = (D^dagger D + fmass^2) src                        [utilities.c:874]
+void DSq(Twist_Fermion *src, Twist_Fermion *dest);
+// (D^dagger D + shift[j]) psim[j] = src for j < Norder, zero initial guess;
+// reads the bound globals Norder and shift[]; returns the iteration count,
+// adds it to the bound total_iters and writes |r|^2 to *size_r
+//                                                          [congrad_multi.c:26]
+int congrad_multi(Twist_Fermion *src, Twist_Fermion **psim, int MaxCG,
+                  Real errormin, Real *size_r);
+// mom -= eps * sum_j amp4[j] dS_F/dU for the bound amp4[], Norder; src is
+// unused, as in the reference; returns eps*|F|/volume      [update_h.c:928]
+double fermion_force(Real eps, Twist_Fermion *src, Twist_Fermion **sol);
+// mom -= kappa eps dS_B/dU (+ det_force when kappa_u1 > 0) [update_h.c:15]
+double gauge_force(Real eps);
+
+// ---------------------------------------------------------------------
+// 2. Binding a reference driver (replaces direct access to its globals)
+// Lattice geometry and temporal fermion BC (setup.c:67-72; PBC = +1 or -1).
+int susy_bind_lattice(int nx, int ny, int nz, int nt, int PBC);
+// The caller's site array: base pointer, sizeof(site), offsetof(site, link)
+// and offsetof(site, mom).  Links are read from it by susy_links_changed();
+// the force entry points update mom in it.
+int susy_bind_sites(void *lattice, size_t site_bytes, size_t link_offset,
+                    size_t mom_offset);
+// Couplings as set in readin (setup.c:445-459): lambda, kappa_u1, bmass,
+// fmass, G.  kappa = NCOL/(4 lambda) and doG = (G > 1e-8) follow from them.
+int susy_bind_couplings(double lambda, double kappa_u1, double bmass,
+                        double fmass, double G);
+// Pointers to the caller's RHMC globals, read at every call:
+// Norder, shift[], amp4[], total_iters (lattice.h:161-164, 110).
+int susy_bind_rhmc(int *Norder, Real **shift, Real **amp4, int *total_iters);
+// Re-read the links from the bound site array and recompute the derived
+// fields (plaqdet, U^-1, DmuUmu, Fmunu) on the device.  Call after every
+// change of the links: where the reference calls compute_plaqdet() and
+// friends (update_o.c:31-34, setup.c:551-554).
+int susy_links_changed(void);
+
+// ---------------------------------------------------------------------
+// 3. Device-resident API
+typedef struct susy_ctx susy_ctx;
+
+typedef struct {
+  int nx, ny, nz, nt, PBC, iseed;
+  int Nroot, Norder;             // Norder must be 15 (tables of setup_rhmc.c)
+  double lambda, kappa_u1, bmass, fmass, G;
+  int niter;                     // max CG iterations
+  double error_per_site;         // CG stops at |r|^2 <= error^2 |src|^2
+  double traj_length;
+  int nstep, nstep_gauge;        // fermion and gauge integrator steps
+} susy_params;
+
+// Precision form of the solver inside a trajectory.
+enum {
+  SUSY_FORM_FP64 = 0,            // plain FP64 everywhere (parity form)
+  SUSY_FORM_FP32 = 1,            // FP32 DSq + CG with FP64 reliable updates
+};
+
+int susy_ncol(void);
+const char *susy_last_error(void);
+// Host-only consistency check of the operator/force term tables (0 = ok).
+int susy_validate_tables(void);
+// CG tuning: residual window of the shifted-system updates (1, 4, 8, 16 or
+// 32 iterations) and, for the FP32 form, iterations between FP64 true
+// residuals.  Process-wide.
+int susy_set_cg_options(int window, int reliable);
+int susy_create(const susy_params *p, susy_ctx **out);
+void susy_destroy(susy_ctx *c);
+int susy_volume(const susy_ctx *c);
+
+// Site order is the reference's (even sites first, layout_hyper_prime.c:157).
+// Gather table g in 0..73 as numbered by the reference (com_vanilla.c,
+// setup_offset.c): out[i] = index of the site gathered to site i.
+int susy_get_gather(susy_ctx *c, int g, int *out);
+
+// Host <-> device transfers of whole fields in the reference's AoS layout:
+// links/momenta = V x 5 matrices; Twist_Fermion fields = V x 16 matrices.
+int susy_set_links(susy_ctx *c, const double *links);   // + derived fields
+int susy_get_links(susy_ctx *c, double *links);
+int susy_set_mom(susy_ctx *c, const double *mom);
+int susy_get_mom(susy_ctx *c, double *mom);
+// MILC binary gauge file (io_lat.c:826-1006), natural order; returns 1 if the
+// file's checksums do not match, 0 if they do.
+int susy_load_config(susy_ctx *c, const char *path);
+int susy_cold_start(susy_ctx *c);
+// Derived fields (AoS): plaqdet 25 x V complex ([a][b][site]), Uinv 5 x V,
+// DmuUmu V, Fmunu 10 x V matrices.  Any pointer may be NULL.
+int susy_get_fields(susy_ctx *c, double *plaqdet, double *uinv, double *dmu,
+                    double *fmunu);
+
+// Device Twist_Fermion vectors owned by the context (SoA in HBM).
+int susy_tf_alloc(susy_ctx *c, int count);          // returns first handle
+int susy_tf_upload(susy_ctx *c, int h, const Twist_Fermion *src);
+int susy_tf_download(susy_ctx *c, int h, Twist_Fermion *dst);
+
+// Resident operations on handles.
+int susy_fermion_op_dev(susy_ctx *c, int h_src, int h_dst, int sign, int form);
+int susy_dsq_dev(susy_ctx *c, int h_src, int h_dst, int form);
+// Multi-shift CG: psim handles h_psim .. h_psim+nshift-1.  Returns iterations
+// (>= 0) or an error code.  form selects the precision form.
+int susy_congrad_multi_dev(susy_ctx *c, int h_src, int h_psim, int nshift,
+                           const double *shifts, int MaxCG, double errormin,
+                           double *size_r, int form);
+// Fermion force from nsol solutions; updates the device momenta.
+int susy_fermion_force_dev(susy_ctx *c, double eps, int h_sol, int nsol,
+                           const double *amps, double *norm);
+int susy_gauge_force_dev(susy_ctx *c, double eps, double *norm);
+int susy_update_u_dev(susy_ctx *c, double eps);
+// Action parts: gauge, bmass, det, mom (FP64, compensated sums).
+int susy_actions_dev(susy_ctx *c, double *out4);
+int susy_fermion_action_dev(susy_ctx *c, int h_src, int h_sol, double *out);
+int susy_plaquette_dev(susy_ctx *c, double *ss, double *st);
+
+// Heatbath pieces of a trajectory, with the reference's per-site generators
+// (grsource.c:44-64, 200-303): fresh Gaussian momenta; a pseudofermion
+// src = (D^dag D)^(1/8 Nroot) g with the solutions of its solve in
+// h_psim .. h_psim+Norder-1.
+int susy_ranmom(susy_ctx *c);
+int susy_grsource_dev(susy_ctx *c, int h_src, int h_psim, int form, int *iters);
+
+// One RHMC trajectory (update_o.c:127-282) in the given form.
+// stats[0..22]: start S, end S, dS, accepted, CG iterations, gauge-force
+// avg/max, fermion-force avg/max for roots 0 and 1, start action parts
+// (gauge, bmass, det, fermion, mom, total), end action parts.
+int susy_update(susy_ctx *c, int form, double *stats);
+
+// Timing of the last call's kernels is available through the profiler; the
+// benchmark measures operator launches with these helpers (HIP events on the
+// library's own stream).  Returns average milliseconds per call.
+int susy_time_fermion_op(susy_ctx *c, int h_src, int h_dst, int form,
+                         int reps, double *ms_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+#endif  // SUSY_HIP_H

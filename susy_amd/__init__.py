@@ -1,0 +1,264 @@
+"""MI355X-native 4d N=4 SYM RHMC hot path (fermion_op / DSq, congrad_multi,
+fermion and gauge forces) behind the reference's C interface.
+
+The compute lives in the HIP libraries susy_amd/lib/libsusy_hip_U<N>.so;
+this package only loads them (ctypes) and exposes the resident API with
+numpy arrays in the reference's AoS layout.  There is no CPU fallback: if the
+library for a gauge group is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+FORM_FP64 = 0
+FORM_FP32 = 1
+NGATHER = 74
+
+
+class SusyParams(ctypes.Structure):
+    """Mirror of susy_params in include/susy_hip.h (input-file parameters of
+    the reference, setup.c:16-119 and 307-556)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("nt", ctypes.c_int), ("PBC", ctypes.c_int), ("iseed", ctypes.c_int),
+                ("Nroot", ctypes.c_int), ("Norder", ctypes.c_int),
+                ("lambda_", ctypes.c_double), ("kappa_u1", ctypes.c_double),
+                ("bmass", ctypes.c_double), ("fmass", ctypes.c_double),
+                ("G", ctypes.c_double), ("niter", ctypes.c_int),
+                ("error_per_site", ctypes.c_double), ("traj_length", ctypes.c_double),
+                ("nstep", ctypes.c_int), ("nstep_gauge", ctypes.c_int)]
+
+
+# every symbol declared in include/susy_hip.h
+EXPORTS = [
+    "fermion_op", "DSq", "congrad_multi", "fermion_force", "gauge_force",
+    "susy_bind_lattice", "susy_bind_sites", "susy_bind_couplings", "susy_bind_rhmc",
+    "susy_links_changed", "susy_ncol", "susy_last_error", "susy_create", "susy_destroy",
+    "susy_volume", "susy_get_gather", "susy_set_links", "susy_get_links", "susy_set_mom",
+    "susy_get_mom", "susy_load_config", "susy_cold_start", "susy_get_fields",
+    "susy_tf_alloc", "susy_tf_upload", "susy_tf_download", "susy_fermion_op_dev",
+    "susy_dsq_dev", "susy_congrad_multi_dev", "susy_fermion_force_dev",
+    "susy_gauge_force_dev", "susy_update_u_dev", "susy_actions_dev",
+    "susy_fermion_action_dev", "susy_plaquette_dev", "susy_update", "susy_time_fermion_op",
+    "susy_set_cg_options", "susy_ranmom", "susy_grsource_dev", "susy_validate_tables",
+]
+
+_libs = {}
+
+
+def lib_path(ncol):
+    return os.path.join(LIBDIR, f"libsusy_hip_U{ncol}.so")
+
+
+def load(ncol):
+    """Load libsusy_hip_U<ncol>.so (raises if it was not built)."""
+    if ncol in _libs:
+        return _libs[ncol]
+    path = lib_path(ncol)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built: run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    dp, ip = P(d), P(i)
+    sig = {
+        "susy_ncol": (i, []), "susy_validate_tables": (i, []), "susy_last_error": (ctypes.c_char_p, []),
+        "susy_create": (i, [P(SusyParams), P(vp)]), "susy_destroy": (None, [vp]),
+        "susy_volume": (i, [vp]), "susy_get_gather": (i, [vp, i, ip]),
+        "susy_set_links": (i, [vp, dp]), "susy_get_links": (i, [vp, dp]),
+        "susy_set_mom": (i, [vp, dp]), "susy_get_mom": (i, [vp, dp]),
+        "susy_load_config": (i, [vp, ctypes.c_char_p]), "susy_cold_start": (i, [vp]),
+        "susy_get_fields": (i, [vp, dp, dp, dp, dp]),
+        "susy_tf_alloc": (i, [vp, i]), "susy_tf_upload": (i, [vp, i, dp]),
+        "susy_tf_download": (i, [vp, i, dp]),
+        "susy_fermion_op_dev": (i, [vp, i, i, i, i]), "susy_dsq_dev": (i, [vp, i, i, i]),
+        "susy_congrad_multi_dev": (i, [vp, i, i, i, dp, i, d, dp, i]),
+        "susy_fermion_force_dev": (i, [vp, d, i, i, dp, dp]),
+        "susy_gauge_force_dev": (i, [vp, d, dp]), "susy_update_u_dev": (i, [vp, d]),
+        "susy_actions_dev": (i, [vp, dp]), "susy_fermion_action_dev": (i, [vp, i, i, dp]),
+        "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
+        "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
+        "susy_set_cg_options": (i, [i, i]), "susy_ranmom": (i, [vp]),
+        "susy_grsource_dev": (i, [vp, i, i, i, ip]),
+        "susy_bind_lattice": (i, [i, i, i, i, i]),
+        "susy_bind_sites": (i, [vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t]),
+        "susy_bind_couplings": (i, [d, d, d, d, d]),
+        "susy_bind_rhmc": (i, [ip, P(dp), P(dp), ip]), "susy_links_changed": (i, []),
+        "fermion_op": (None, [vp, vp, i]), "DSq": (None, [vp, vp]),
+        "congrad_multi": (i, [vp, P(vp), i, d, dp]),
+        "fermion_force": (d, [d, vp, P(vp)]), "gauge_force": (d, [d]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if L.susy_ncol() != ncol:
+        raise RuntimeError(f"{path} reports NCOL={L.susy_ncol()}")
+    _libs[ncol] = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Lattice:
+    """Device-resident lattice: links, momenta, derived fields and
+    Twist_Fermion vectors in HBM (one susy_ctx)."""
+
+    def __init__(self, ncol, dims=(4, 4, 4, 4), PBC=-1, iseed=41, Nroot=1, Norder=15,
+                 lam=1.0, kappa_u1=0.0, bmass=0.5, fmass=0.0, G=0.05, niter=5000,
+                 error_per_site=1e-5, traj_length=1.0, nstep=10, nstep_gauge=10):
+        self.ncol = ncol
+        self.L = load(ncol)
+        p = SusyParams(*dims, PBC, iseed, Nroot, Norder, lam, kappa_u1, bmass, fmass, G,
+                       niter, error_per_site, traj_length, nstep, nstep_gauge)
+        self.params = p
+        h = ctypes.c_void_p()
+        self._check(self.L.susy_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.V = self.L.susy_volume(h)
+        self.nn = ncol * ncol
+
+    def _check(self, r):
+        if r < 0:
+            raise RuntimeError(f"susy_hip error {r}: {self.L.susy_last_error().decode()}")
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.susy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- shapes (reference AoS layout, complex128)
+    def tf_shape(self):
+        return (self.V, 16, self.ncol, self.ncol)
+
+    def link_shape(self):
+        return (self.V, 5, self.ncol, self.ncol)
+
+    def gather(self, g):
+        out = np.zeros(self.V, dtype=np.int32)
+        self._check(self.L.susy_get_gather(self.h, g, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def set_links(self, links):
+        a = np.ascontiguousarray(links, dtype=np.complex128)
+        self._check(self.L.susy_set_links(self.h, _dp(a)))
+
+    def get_links(self):
+        a = np.zeros(self.link_shape(), dtype=np.complex128)
+        self._check(self.L.susy_get_links(self.h, _dp(a)))
+        return a
+
+    def set_mom(self, mom):
+        a = np.ascontiguousarray(mom, dtype=np.complex128)
+        self._check(self.L.susy_set_mom(self.h, _dp(a)))
+
+    def get_mom(self):
+        a = np.zeros(self.link_shape(), dtype=np.complex128)
+        self._check(self.L.susy_get_mom(self.h, _dp(a)))
+        return a
+
+    def load_config(self, path):
+        return self._check(self.L.susy_load_config(self.h, path.encode()))
+
+    def fields(self):
+        V, n = self.V, self.ncol
+        pd = np.zeros((5, 5, V), dtype=np.complex128)
+        ui = np.zeros((V, 5, n, n), dtype=np.complex128)
+        dm = np.zeros((V, 1, n, n), dtype=np.complex128)
+        fm = np.zeros((10, V, 1, n, n), dtype=np.complex128)
+        self._check(self.L.susy_get_fields(self.h, _dp(pd), _dp(ui), _dp(dm), _dp(fm)))
+        return pd, ui, dm[:, 0], fm[:, :, 0]
+
+    def tf_alloc(self, count=1):
+        return self._check(self.L.susy_tf_alloc(self.h, count))
+
+    def tf_upload(self, h, tf):
+        a = np.ascontiguousarray(tf, dtype=np.complex128)
+        assert a.shape == self.tf_shape()
+        self._check(self.L.susy_tf_upload(self.h, h, _dp(a)))
+
+    def tf_download(self, h):
+        a = np.zeros(self.tf_shape(), dtype=np.complex128)
+        self._check(self.L.susy_tf_download(self.h, h, _dp(a)))
+        return a
+
+    def fermion_op(self, hs, hd, sign, form=FORM_FP64):
+        self._check(self.L.susy_fermion_op_dev(self.h, hs, hd, sign, form))
+
+    def dsq(self, hs, hd, form=FORM_FP64):
+        self._check(self.L.susy_dsq_dev(self.h, hs, hd, form))
+
+    def congrad_multi(self, hs, hpsim, shifts, maxcg, errormin, form=FORM_FP64):
+        sh = np.ascontiguousarray(shifts, dtype=np.float64)
+        sz = ctypes.c_double()
+        it = self._check(self.L.susy_congrad_multi_dev(
+            self.h, hs, hpsim, len(sh), _dp(sh), maxcg, errormin, ctypes.byref(sz), form))
+        return it, sz.value
+
+    def fermion_force(self, eps, hsol, amps):
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        nm = ctypes.c_double()
+        self._check(self.L.susy_fermion_force_dev(self.h, eps, hsol, len(a), _dp(a),
+                                                  ctypes.byref(nm)))
+        return nm.value
+
+    def gauge_force(self, eps):
+        nm = ctypes.c_double()
+        self._check(self.L.susy_gauge_force_dev(self.h, eps, ctypes.byref(nm)))
+        return nm.value
+
+    def update_u(self, eps):
+        self._check(self.L.susy_update_u_dev(self.h, eps))
+
+    def actions(self):
+        out = np.zeros(4)
+        self._check(self.L.susy_actions_dev(self.h, _dp(out)))
+        return out
+
+    def fermion_action(self, hs, hsol):
+        out = ctypes.c_double()
+        self._check(self.L.susy_fermion_action_dev(self.h, hs, hsol, ctypes.byref(out)))
+        return out.value
+
+    def plaquette(self):
+        ss, st = ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.susy_plaquette_dev(self.h, ctypes.byref(ss), ctypes.byref(st)))
+        return ss.value, st.value
+
+    def update(self, form=FORM_FP64):
+        """One RHMC trajectory; returns the stats vector (see susy_update)."""
+        st = np.zeros(32)
+        self._check(self.L.susy_update(self.h, form, _dp(st)))
+        return st
+
+    def time_fermion_op(self, hs, hd, form=FORM_FP64, reps=20):
+        ms = ctypes.c_double()
+        self._check(self.L.susy_time_fermion_op(self.h, hs, hd, form, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def ranmom(self):
+        self._check(self.L.susy_ranmom(self.h))
+
+    def grsource(self, hsrc, hpsim, form=FORM_FP64):
+        it = ctypes.c_int()
+        self._check(self.L.susy_grsource_dev(self.h, hsrc, hpsim, form, ctypes.byref(it)))
+        return it.value
+
+
+def set_cg_options(ncol, window=16, reliable=50):
+    r = load(ncol).susy_set_cg_options(window, reliable)
+    if r < 0:
+        raise RuntimeError(load(ncol).susy_last_error().decode())

@@ -1,0 +1,973 @@
+// C ABI of libsusy_hip_U<N>: context management, resident operations, the
+// RHMC trajectory driver and the drop-in replacements of the reference's
+// fermion_op / DSq / congrad_multi / fermion_force / gauge_force.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+
+#include "../../include/susy_rhmc15.h"
+#include "susy_internal.h"
+
+#define RET(x)               \
+  do {                       \
+    int _r = (x);            \
+    if (_r < 0) return _r;   \
+  } while (0)
+
+extern "C" int susy_ncol(void) { return NCOL; }
+extern "C" int susy_volume(const susy_ctx *c) { return c ? c->V : SUSY_E_STATE; }
+
+static bool g_tables_ready = false;
+
+// Every gather id used by the fused kernels must name one of the 74 tables.
+static int validate_tables(const PlaqTerms &pt, const ForceTerms &ft) {
+  auto ok = [](int g) { return g >= 0 && g < NGATHER; };
+  for (int p = 0; p < NPLAQ; p++)
+    for (int n = 0; n < 3; n++)
+      if (!ok(pt.pp_g1[p][n]) || !ok(pt.pp_g2[p][n]) || !ok(pt.pm_g1[p][n]) ||
+          !ok(pt.pm_g2[p][n]) || pt.pp_c[p][n] < 0 || pt.pp_c[p][n] >= NUMLINK ||
+          pt.pm_c[p][n] < 0 || pt.pm_c[p][n] >= NUMLINK || pt.pp_de[p][n] < 0 ||
+          pt.pp_de[p][n] >= NPLAQ || pt.pm_ab[p][n] < 0 || pt.pm_ab[p][n] >= NPLAQ) {
+        set_error("operator term table entry (%d,%d) invalid", p, n);
+        return SUSY_E_STATE;
+      }
+  for (int c = 0; c < NUMLINK; c++)
+    for (int n = 0; n < 6; n++)
+      if (!ok(ft.g_abc[c][n]) || !ok(ft.g_mab[c][n]) || ft.ab[c][n] < 0 ||
+          ft.ab[c][n] >= NPLAQ || ft.de[c][n] < 0 || ft.de[c][n] >= NPLAQ) {
+        set_error("force term table entry (%d,%d) invalid", c, n);
+        return SUSY_E_STATE;
+      }
+  return 0;
+}
+
+// Host-only check of the term tables (callable without a GPU).
+extern "C" int susy_validate_tables(void) {
+  Geometry g;
+  build_geometry(g, 2, 2, 2, 2);
+  PlaqTerms pt;
+  ForceTerms ft;
+  build_plaq_terms(g.q_offset, pt);
+  build_force_terms(g.q_offset, ft);
+  return validate_tables(pt, ft);
+}
+
+static void free_ctx(susy_ctx *c) {
+  void *ptrs[] = {c->d_nbr, c->d_t, c->d_links, c->d_mom, c->d_old_links, c->d_old_mom,
+                  c->d_uinv, c->d_det, c->d_pdet, c->d_zw, c->d_dmu, c->d_fmunu,
+                  c->d_force, c->f_links, c->f_uinv, c->f_pdet, c->d_aux, c->f_aux,
+                  c->d_tmp, c->f_tmp, c->d_boundary, c->d_part, c->d_scal, c->cg_ws,
+                  c->cg_t64, c->d_coef, c->f_in, c->f_out};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (double2 *p : c->tf)
+    if (p) (void)hipFree(p);
+  for (double2 *p : c->traj_pool)
+    if (p) (void)hipFree(p);
+  if (c->h_coef) (void)hipHostFree(c->h_coef);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+extern "C" void susy_destroy(susy_ctx *c) {
+  if (c) {
+    (void)hipStreamSynchronize(c->s);
+    free_ctx(c);
+  }
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return SUSY_E_NOMEM;
+  }
+  return 0;
+}
+
+static int ctx_setup(susy_ctx *c) {
+  const susy_params &p = c->p;
+  build_geometry(c->geo, p.nx, p.ny, p.nz, p.nt);
+  const int V = c->V = c->geo.V;
+  c->kappa = (double)NCOL * 0.25 / p.lambda;
+  c->doG = p.G > IMAG_TOL;
+  int r = p.Nroot - 1;
+  c->ampdeg4 = susy_rhmc15_ampdeg4[r];
+  c->ampdeg8 = susy_rhmc15_ampdeg8[r];
+  for (int j = 0; j < 15; j++) {
+    c->amp4[j] = susy_rhmc15_amp4[r][j];
+    c->shift4[j] = susy_rhmc15_shift4[r][j];
+    c->amp8[j] = susy_rhmc15_amp8[r][j];
+    c->shift8[j] = susy_rhmc15_shift8[r][j];
+  }
+  build_lambda(c->lambda_gen);
+  if (!g_tables_ready) {
+    PlaqTerms pt;
+    ForceTerms ft;
+    build_plaq_terms(c->geo.q_offset, pt);
+    build_force_terms(c->geo.q_offset, ft);
+    RET(validate_tables(pt, ft));
+    RET(dslash_init_tables(pt));
+    RET(force_init_tables(ft));
+    g_tables_ready = true;
+  }
+  HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  const size_t mat5 = (size_t)NUMLINK * NN * V, tfn = (size_t)NTF * NN * V;
+  RET(dalloc(&c->d_nbr, (size_t)NGATHER * V));
+  RET(dalloc(&c->d_t, (size_t)V));
+  RET(dalloc(&c->d_links, mat5));
+  RET(dalloc(&c->d_mom, mat5));
+  RET(dalloc(&c->d_old_links, mat5));
+  RET(dalloc(&c->d_old_mom, mat5));
+  RET(dalloc(&c->d_uinv, mat5));
+  RET(dalloc(&c->d_force, mat5));
+  RET(dalloc(&c->d_det, (size_t)NUMLINK * V));
+  RET(dalloc(&c->d_pdet, (size_t)NUMLINK * NUMLINK * V));
+  RET(dalloc(&c->d_zw, (size_t)NUMLINK * NUMLINK * V));
+  RET(dalloc(&c->d_dmu, (size_t)NN * V));
+  RET(dalloc(&c->d_fmunu, (size_t)NPLAQ * NN * V));
+  RET(dalloc(&c->f_links, mat5));
+  RET(dalloc(&c->f_uinv, mat5));
+  RET(dalloc(&c->f_pdet, (size_t)NUMLINK * NUMLINK * V));
+  RET(dalloc(&c->d_aux, (size_t)12 * V));
+  RET(dalloc(&c->f_aux, (size_t)6 * V));
+  RET(dalloc(&c->d_tmp, tfn));
+  RET(dalloc(&c->d_boundary, tfn));
+  int chunks = (V + 127) / 128;
+  int ds_blocks = NTF * ((chunks + 7) / 8 * 8);
+  c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
+  RET(dalloc(&c->d_part, (size_t)c->npart * 2));
+  RET(dalloc(&c->d_scal, (size_t)64));
+  HIPCHK(hipMemcpy(c->d_nbr, c->geo.gathers.data(), sizeof(int) * NGATHER * V,
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_t, c->geo.t.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_mom, 0, sizeof(double2) * mat5));
+  // generators: one per site (index = lexicographic site number), one for
+  // the node (index = volume) (make_lattice.c:48, setup.c:246)
+  c->site_prn.resize(V);
+  for (int i = 0; i < V; i++) {
+    int lex = c->geo.x[i] + p.nx * (c->geo.y[i] + p.ny * (c->geo.z[i] + p.nz * c->geo.t[i]));
+    prn_init(&c->site_prn[i], p.iseed, lex);
+  }
+  prn_init(&c->node_prn, p.iseed, V);
+  return susy_cold_start(c);
+}
+
+extern "C" int susy_create(const susy_params *p, susy_ctx **out) {
+  if (!p || !out) return SUSY_E_ARG;
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1 || p->nt < 1 ||
+      (p->nx * p->ny * p->nz * p->nt) % 2 != 0) {
+    set_error("lattice %dx%dx%dx%d: need an even number of sites", p->nx, p->ny, p->nz, p->nt);
+    return SUSY_E_ARG;
+  }
+  if (p->PBC != 1 && p->PBC != -1) {
+    set_error("PBC must be +1 or -1 (setup_offset.c:190-199)");
+    return SUSY_E_ARG;
+  }
+  if (p->Norder != SUSY_RHMC15_NORDER || (p->Nroot != 1 && p->Nroot != 2)) {
+    set_error("only Norder 15 with Nroot 1 or 2 is tabulated");
+    return SUSY_E_ARG;
+  }
+  if (p->lambda <= 0) {
+    set_error("lambda must be positive");
+    return SUSY_E_ARG;
+  }
+  susy_ctx *c = new susy_ctx();
+  c->p = *p;
+  int r = ctx_setup(c);
+  if (r) {
+    free_ctx(c);
+    return r;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int susy_get_gather(susy_ctx *c, int g, int *out) {
+  if (!c) return SUSY_E_STATE;
+  if (g < 0 || g >= NGATHER || !out) return SUSY_E_ARG;
+  memcpy(out, &c->geo.gathers[(size_t)g * c->V], sizeof(int) * c->V);
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// Field transfers (AoS on the host)
+static int upload_aos(susy_ctx *c, const double *h, double2 *d, int nmat) {
+  size_t bytes = (size_t)nmat * NN * c->V * sizeof(double2);
+  HIPCHK(hipMemcpyAsync(c->d_boundary, h, bytes, hipMemcpyHostToDevice, c->s));
+  RET(aos_to_soa(c, (const double *)c->d_boundary, d, nmat));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+static int download_aos(susy_ctx *c, const double2 *d, double *h, int nmat) {
+  size_t bytes = (size_t)nmat * NN * c->V * sizeof(double2);
+  RET(soa_to_aos(c, d, (double *)c->d_boundary, nmat));
+  HIPCHK(hipMemcpyAsync(h, c->d_boundary, bytes, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+extern "C" int susy_set_links(susy_ctx *c, const double *links) {
+  if (!c) return SUSY_E_STATE;
+  RET(upload_aos(c, links, c->d_links, NUMLINK));
+  RET(fields_recompute(c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+extern "C" int susy_get_links(susy_ctx *c, double *links) {
+  if (!c) return SUSY_E_STATE;
+  return download_aos(c, c->d_links, links, NUMLINK);
+}
+extern "C" int susy_set_mom(susy_ctx *c, const double *mom) {
+  if (!c) return SUSY_E_STATE;
+  return upload_aos(c, mom, c->d_mom, NUMLINK);
+}
+extern "C" int susy_get_mom(susy_ctx *c, double *mom) {
+  if (!c) return SUSY_E_STATE;
+  return download_aos(c, c->d_mom, mom, NUMLINK);
+}
+
+// unit links (io_helpers.c:95-111)
+extern "C" int susy_cold_start(susy_ctx *c) {
+  if (!c) return SUSY_E_STATE;
+  std::vector<double> h((size_t)c->V * NUMLINK * NN * 2, 0.0);
+  for (size_t k = 0; k < (size_t)c->V * NUMLINK; k++)
+    for (int i = 0; i < NCOL; i++) h[(k * NN + i * NCOL + i) * 2] = 1.0;
+  return susy_set_links(c, h.data());
+}
+
+extern "C" int susy_get_fields(susy_ctx *c, double *plaqdet, double *uinv, double *dmu,
+                               double *fmunu) {
+  if (!c) return SUSY_E_STATE;
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (plaqdet)
+    HIPCHK(hipMemcpy(plaqdet, c->d_pdet, sizeof(double2) * NUMLINK * NUMLINK * c->V,
+                     hipMemcpyDeviceToHost));
+  if (uinv) RET(download_aos(c, c->d_uinv, uinv, NUMLINK));
+  if (dmu) RET(download_aos(c, c->d_dmu, dmu, 1));
+  if (fmunu) {
+    // Fmunu as 10 separate fields of one matrix per site
+    std::vector<double> tmp((size_t)c->V * NN * 2);
+    for (int p = 0; p < NPLAQ; p++) {
+      RET(download_aos(c, c->d_fmunu + (size_t)p * NN * c->V, tmp.data(), 1));
+      memcpy(fmunu + (size_t)p * c->V * NN * 2, tmp.data(), tmp.size() * sizeof(double));
+    }
+  }
+  return 0;
+}
+
+// MILC binary gauge file, natural order, FP32 links (io_lat.c:826-1006,
+// io_lat_utils.c:519-618), with the sum29/sum31 checksums.
+static uint32_t bswap32(uint32_t u) {
+  return (u >> 24) | ((u >> 8) & 0xff00) | ((u << 8) & 0xff0000) | (u << 24);
+}
+extern "C" int susy_load_config(susy_ctx *c, const char *path) {
+  if (!c || !path) return SUSY_E_ARG;
+  FILE *fp = fopen(path, "rb");
+  if (!fp) {
+    set_error("cannot open %s", path);
+    return SUSY_E_ARG;
+  }
+  uint32_t hdr[5], stamp_order[17], sums[2];
+  int swap = 0;
+  bool ok = fread(hdr, 4, 5, fp) == 5;
+  if (ok && hdr[0] != 20103) {
+    swap = bswap32(hdr[0]) == 20103;
+    if (!swap) ok = false;
+  }
+  ok = ok && fread(stamp_order, 4, 17, fp) == 17 && fread(sums, 4, 2, fp) == 2;
+  if (!ok) {
+    fclose(fp);
+    set_error("%s: not a MILC gauge file", path);
+    return SUSY_E_ARG;
+  }
+  if (swap) {
+    for (int k = 1; k < 5; k++) hdr[k] = bswap32(hdr[k]);
+    stamp_order[16] = bswap32(stamp_order[16]);
+    sums[0] = bswap32(sums[0]);
+    sums[1] = bswap32(sums[1]);
+  }
+  if ((int)hdr[1] != c->p.nx || (int)hdr[2] != c->p.ny || (int)hdr[3] != c->p.nz ||
+      (int)hdr[4] != c->p.nt || stamp_order[16] != 0) {
+    fclose(fp);
+    set_error("%s: dimensions or site order do not match", path);
+    return SUSY_E_ARG;
+  }
+  const int V = c->V;
+  const size_t nw = (size_t)NUMLINK * NN * 2;
+  std::vector<float> buf(nw);
+  std::vector<double> links((size_t)V * nw);
+  uint32_t c29 = 0, c31 = 0;
+  int r29 = 0, r31 = 0;
+  for (int lex = 0; lex < V; lex++) {
+    if (fread(buf.data(), sizeof(float), nw, fp) != nw) {
+      fclose(fp);
+      set_error("%s: short read", path);
+      return SUSY_E_ARG;
+    }
+    uint32_t *w = (uint32_t *)buf.data();
+    for (size_t k = 0; k < nw; k++) {
+      if (swap) w[k] = bswap32(w[k]);
+      uint32_t v = w[k];
+      c29 ^= r29 ? (v << r29 | v >> (32 - r29)) : v;
+      c31 ^= r31 ? (v << r31 | v >> (32 - r31)) : v;
+      if (++r29 >= 29) r29 = 0;
+      if (++r31 >= 31) r31 = 0;
+    }
+    int x = lex % c->p.nx, y = (lex / c->p.nx) % c->p.ny;
+    int z = (lex / (c->p.nx * c->p.ny)) % c->p.nz, t = lex / (c->p.nx * c->p.ny * c->p.nz);
+    int i = site_index(c->geo, x, y, z, t);
+    for (size_t k = 0; k < nw; k++) links[(size_t)i * nw + k] = (double)buf[k];
+  }
+  fclose(fp);
+  RET(susy_set_links(c, links.data()));
+  return (c29 == sums[0] && c31 == sums[1]) ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------
+// Twist_Fermion handles
+extern "C" int susy_tf_alloc(susy_ctx *c, int count) {
+  if (!c) return SUSY_E_STATE;
+  if (count < 1) return SUSY_E_ARG;
+  int first = (int)c->tf.size();
+  for (int k = 0; k < count; k++) {
+    double2 *p = nullptr;
+    RET(dalloc(&p, tf_elems(c)));
+    HIPCHK(hipMemset(p, 0, sizeof(double2) * tf_elems(c)));
+    c->tf.push_back(p);
+  }
+  return first;
+}
+static double2 *tfh(susy_ctx *c, int h) {
+  if (!c || h < 0 || h >= (int)c->tf.size()) {
+    set_error("bad Twist_Fermion handle %d", h);
+    return nullptr;
+  }
+  return c->tf[h];
+}
+extern "C" int susy_tf_upload(susy_ctx *c, int h, const Twist_Fermion *src) {
+  double2 *d = tfh(c, h);
+  if (!d) return SUSY_E_ARG;
+  return upload_aos(c, (const double *)src, d, NTF);
+}
+extern "C" int susy_tf_download(susy_ctx *c, int h, Twist_Fermion *dst) {
+  double2 *d = tfh(c, h);
+  if (!d) return SUSY_E_ARG;
+  return download_aos(c, d, (double *)dst, NTF);
+}
+
+// FP32 staging for the FP32 operator form (c->f_in, c->f_out, c->f_tmp)
+__global__ void k_to32(const double2 *__restrict__ a, float2 *__restrict__ b, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+__global__ void k_to64(const float2 *__restrict__ a, double2 *__restrict__ b, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+static int f32_stage(susy_ctx *c) {
+  size_t n = tf_elems(c);
+  if (!c->f_in) RET(dalloc(&c->f_in, n));
+  if (!c->f_out) RET(dalloc(&c->f_out, n));
+  if (!c->f_tmp) RET(dalloc(&c->f_tmp, n));
+  return 0;
+}
+
+extern "C" int susy_fermion_op_dev(susy_ctx *c, int hs, int hd, int sign, int form) {
+  double2 *s = tfh(c, hs), *d = tfh(c, hd);
+  if (!s || !d) return SUSY_E_ARG;
+  if (form == SUSY_FORM_FP32) {
+    RET(f32_stage(c));
+    size_t n = tf_elems(c);
+    hipLaunchKernelGGL(k_to32, dim3((n + 255) / 256), dim3(256), 0, c->s, s, c->f_in, n);
+    RET(dslash32(c, c->f_in, c->f_out, sign, nullptr, 0.f, nullptr, nullptr));
+    hipLaunchKernelGGL(k_to64, dim3((n + 255) / 256), dim3(256), 0, c->s, c->f_out, d, n);
+  } else {
+    RET(dslash64(c, s, d, sign, nullptr, 0.0, nullptr, nullptr));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+// dest = D^dag D src + fmass^2 src (utilities.c:874-885)
+static int dsq64(susy_ctx *c, const double2 *s, double2 *d) {
+  double f2 = c->p.fmass > IMAG_TOL ? c->p.fmass * c->p.fmass : 0.0;
+  RET(dslash64(c, s, c->d_tmp, 1, nullptr, 0.0, nullptr, nullptr));
+  int nb;
+  return dslash64(c, c->d_tmp, d, -1, s, f2, c->d_part, &nb);
+}
+extern "C" int susy_dsq_dev(susy_ctx *c, int hs, int hd, int form) {
+  double2 *s = tfh(c, hs), *d = tfh(c, hd);
+  if (!s || !d) return SUSY_E_ARG;
+  if (form == SUSY_FORM_FP32) {
+    RET(f32_stage(c));
+    size_t n = tf_elems(c);
+    float f2 = c->p.fmass > IMAG_TOL ? (float)(c->p.fmass * c->p.fmass) : 0.f;
+    hipLaunchKernelGGL(k_to32, dim3((n + 255) / 256), dim3(256), 0, c->s, s, c->f_in, n);
+    RET(dslash32(c, c->f_in, c->f_tmp, 1, nullptr, 0.f, nullptr, nullptr));
+    int nb;
+    RET(dslash32(c, c->f_tmp, c->f_out, -1, c->f_in, f2, c->d_part, &nb));
+    hipLaunchKernelGGL(k_to64, dim3((n + 255) / 256), dim3(256), 0, c->s, c->f_out, d, n);
+  } else {
+    RET(dsq64(c, s, d));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+
+extern "C" int susy_congrad_multi_dev(susy_ctx *c, int hs, int hp, int nshift,
+                                      const double *shifts, int MaxCG, double errormin,
+                                      double *size_r, int form) {
+  double2 *s = tfh(c, hs);
+  if (!s || nshift < 1 || hp < 0 || hp + nshift > (int)c->tf.size()) return SUSY_E_ARG;
+  std::vector<double2 *> ps(nshift);
+  for (int j = 0; j < nshift; j++) ps[j] = c->tf[hp + j];
+  double sr = 0;
+  int it = cg_multi(c, s, ps.data(), nshift, shifts, MaxCG, errormin, &sr, form);
+  if (size_r) *size_r = sr;
+  return it;
+}
+
+extern "C" int susy_fermion_force_dev(susy_ctx *c, double eps, int hsol, int nsol,
+                                      const double *amps, double *norm) {
+  if (!c || hsol < 0 || nsol < 1 || hsol + nsol > (int)c->tf.size()) return SUSY_E_ARG;
+  std::vector<double2 *> ps(nsol);
+  for (int j = 0; j < nsol; j++) ps[j] = c->tf[hsol + j];
+  RET(fermion_force_dev(c, eps, ps.data(), nsol, amps, norm));
+  return 0;
+}
+extern "C" int susy_gauge_force_dev(susy_ctx *c, double eps, double *norm) {
+  if (!c) return SUSY_E_STATE;
+  return gauge_force_dev(c, eps, norm);
+}
+extern "C" int susy_update_u_dev(susy_ctx *c, double eps) {
+  if (!c) return SUSY_E_STATE;
+  RET(update_u_dev(c, eps));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+extern "C" int susy_actions_dev(susy_ctx *c, double *out4) {
+  if (!c) return SUSY_E_STATE;
+  return actions_dev(c, out4);
+}
+// sum_x [ampdeg4 |src|^2 + sum_j amp4_j Re<src, sol_j>] (action.c:243-268)
+static int fermion_action_ptr(susy_ctx *c, const double2 *src, double2 *const *sol,
+                              double *out) {
+  double s = 0, v;
+  RET(tf_dot_re(c, src, src, &v));
+  s += c->ampdeg4 * v;
+  for (int j = 0; j < c->p.Norder; j++) {
+    RET(tf_dot_re(c, src, sol[j], &v));
+    s += c->amp4[j] * v;
+  }
+  *out = s;
+  return 0;
+}
+extern "C" int susy_fermion_action_dev(susy_ctx *c, int hs, int hsol, double *out) {
+  double2 *s = tfh(c, hs);
+  if (!s || hsol < 0 || hsol + c->p.Norder > (int)c->tf.size()) return SUSY_E_ARG;
+  return fermion_action_ptr(c, s, &c->tf[hsol], out);
+}
+extern "C" int susy_plaquette_dev(susy_ctx *c, double *ss, double *st) {
+  if (!c) return SUSY_E_STATE;
+  return plaquette_dev(c, ss, st);
+}
+
+extern "C" int susy_time_fermion_op(susy_ctx *c, int hs, int hd, int form, int reps,
+                                    double *ms) {
+  double2 *s = tfh(c, hs), *d = tfh(c, hd);
+  if (!s || !d || reps < 1) return SUSY_E_ARG;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  if (form == SUSY_FORM_FP32) {
+    RET(f32_stage(c));
+    size_t n = tf_elems(c);
+    hipLaunchKernelGGL(k_to32, dim3((n + 255) / 256), dim3(256), 0, c->s, s, c->f_in, n);
+    RET(dslash32(c, c->f_in, c->f_out, 1, nullptr, 0.f, nullptr, nullptr));
+    HIPCHK(hipEventRecord(e0, c->s));
+    for (int k = 0; k < reps; k++)
+      RET(dslash32(c, c->f_in, c->f_out, 1, nullptr, 0.f, nullptr, nullptr));
+    HIPCHK(hipEventRecord(e1, c->s));
+  } else {
+    RET(dslash64(c, s, d, 1, nullptr, 0.0, nullptr, nullptr));
+    HIPCHK(hipEventRecord(e0, c->s));
+    for (int k = 0; k < reps; k++) RET(dslash64(c, s, d, 1, nullptr, 0.0, nullptr, nullptr));
+    HIPCHK(hipEventRecord(e1, c->s));
+  }
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// Heatbath on the host with the reference's generators, so that momenta and
+// pseudofermions are bit-identical to the reference's (grsource.c:44-64,
+// 200-303).  Each site draws an even number of Gaussians, so the shared
+// Box-Muller pair cache of gaussian_rand_no (gaussrand.c:12-32) is empty at
+// every site boundary and sites can be generated in parallel.
+struct Gauss {
+  int iset = 0;
+  double gset = 0;
+  double next(Prn *p) {
+    if (iset == 0) {
+      double v1, v2, r;
+      do {
+        v1 = 2.0 * prn_next(p) - 1.0;
+        v2 = 2.0 * prn_next(p) - 1.0;
+        r = v1 * v1 + v2 * v2;
+      } while (r >= 1.0);
+      double fac = sqrt(-log(r) / r);
+      gset = v1 * fac;
+      iset = 1;
+      return v2 * fac;
+    }
+    iset = 0;
+    return gset;
+  }
+};
+
+template <typename F>
+static void parallel_sites(int V, F f) {
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (V < 4096) nt = 1;
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < nt; k++)
+    th.emplace_back([&, k]() {
+      for (int i = (int)((size_t)V * k / nt); i < (int)((size_t)V * (k + 1) / nt); i++) f(i);
+    });
+  for (auto &t : th) t.join();
+}
+
+// out (AoS, nmat matrices per site) = sum_j (g1 + i g2) Lambda_j, drawn in the
+// reference's order: generator j outer, matrix k inner.
+static void heatbath_fill(susy_ctx *c, double *out, int nmat, bool j_outer) {
+  const double *lam = c->lambda_gen.data();
+  parallel_sites(c->V, [&](int i) {
+    Gauss g;
+    Prn *p = &c->site_prn[i];
+    double *o = out + (size_t)i * nmat * NN * 2;
+    memset(o, 0, sizeof(double) * nmat * NN * 2);
+    auto add = [&](int k, int j) {
+      double gr = g.next(p), gi = g.next(p);
+      const double *L = lam + (size_t)j * NN * 2;
+      double *m = o + (size_t)k * NN * 2;
+      for (int e = 0; e < NN; e++) {
+        m[2 * e] += gr * L[2 * e] - gi * L[2 * e + 1];
+        m[2 * e + 1] += gr * L[2 * e + 1] + gi * L[2 * e];
+      }
+    };
+    if (j_outer) {
+      for (int j = 0; j < NN; j++)
+        for (int k = 0; k < nmat; k++) add(k, j);
+    } else {
+      for (int k = 0; k < nmat; k++)
+        for (int j = 0; j < NN; j++) add(k, j);
+    }
+  });
+}
+
+static int ranmom(susy_ctx *c) {
+  std::vector<double> h((size_t)c->V * NUMLINK * NN * 2);
+  heatbath_fill(c, h.data(), NUMLINK, false);
+  return upload_aos(c, h.data(), c->d_mom, NUMLINK);
+}
+
+extern "C" int susy_ranmom(susy_ctx *c) {
+  if (!c) return SUSY_E_STATE;
+  return ranmom(c);
+}
+
+// src = (D^dag D)^(1/8 Nroot) g: g Gaussian, then the heatbath rational
+// function through one multi-shift solve with shift8
+static int grsource(susy_ctx *c, double2 *src, double2 *const *psim, int form, int *iters) {
+  std::vector<double> h((size_t)c->V * NTF * NN * 2);
+  heatbath_fill(c, h.data(), NTF, true);
+  RET(upload_aos(c, h.data(), src, NTF));
+  double sr;
+  int it = cg_multi(c, src, psim, c->p.Norder, c->shift8, c->p.niter,
+                    c->p.error_per_site * c->p.error_per_site, &sr, form);
+  if (it < 0) return it;
+  *iters += it;
+  RET(tf_axpby(c, src, c->ampdeg8, psim[0], c->amp8[0]));
+  for (int j = 1; j < c->p.Norder; j++) RET(tf_axpby(c, src, 1.0, psim[j], c->amp8[j]));
+  return 0;
+}
+
+extern "C" int susy_grsource_dev(susy_ctx *c, int hsrc, int hpsim, int form, int *iters) {
+  double2 *s = tfh(c, hsrc);
+  if (!s || hpsim < 0 || hpsim + c->p.Norder > (int)c->tf.size()) return SUSY_E_ARG;
+  int it = 0;
+  RET(grsource(c, s, &c->tf[hpsim], form, &it));
+  if (iters) *iters = it;
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// One trajectory (update_o.c:20-282)
+struct Traj {
+  susy_ctx *c;
+  int form;
+  std::vector<double2 *> gsrc;
+  std::vector<std::vector<double2 *>> psim;
+  double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
+};
+
+static const double OM_LAMBDA = 0.193, OM_TWO_LAMBDA = 0.386, OM_LAMBDA_MID = 0.614;
+
+static int update_gauge_step(Traj &T, double eps, double *norm) {
+  susy_ctx *c = T.c;
+  int n = c->p.nstep_gauge;
+  double nm, acc;
+  RET(gauge_force_dev(c, eps * OM_LAMBDA, &acc));
+  for (int i = 1; i <= n; i++) {
+    RET(update_u_dev(c, 0.5 * eps));
+    RET(gauge_force_dev(c, eps * OM_LAMBDA_MID, &nm));
+    acc += nm;
+    RET(update_u_dev(c, 0.5 * eps));
+    RET(gauge_force_dev(c, eps * (i < n ? OM_TWO_LAMBDA : OM_LAMBDA), &nm));
+    acc += nm;
+  }
+  *norm = acc / n;
+  return 0;
+}
+
+static int traj_force(Traj &T, int root, double eps) {
+  double nm;
+  RET(fermion_force_dev(T.c, eps, T.psim[root].data(), T.c->p.Norder, T.c->amp4, &nm));
+  T.fnorm[root] += nm;
+  if (nm > T.max_ff[root]) T.max_ff[root] = nm;
+  return 0;
+}
+
+static int traj_cg(Traj &T, int root, int *iters) {
+  susy_ctx *c = T.c;
+  double sr;
+  int it = cg_multi(c, T.gsrc[root], T.psim[root].data(), c->p.Norder, c->shift4, c->p.niter,
+                    c->p.error_per_site * c->p.error_per_site, &sr, T.form);
+  if (it < 0) return it;
+  *iters += it;
+  return 0;
+}
+
+static int update_step(Traj &T, int *iters) {
+  susy_ctx *c = T.c;
+  const double f_eps = c->p.traj_length / (double)c->p.nstep;
+  const double g_eps = f_eps / (double)(2.0 * c->p.nstep_gauge);
+  double tr;
+  for (int n = 0; n < c->p.Nroot; n++) RET(traj_force(T, n, f_eps * OM_LAMBDA));
+  for (int s = 1; s <= c->p.nstep; s++) {
+    RET(update_gauge_step(T, g_eps, &tr));
+    T.gnorm += tr;
+    if (tr > T.max_gf) T.max_gf = tr;
+    for (int n = 0; n < c->p.Nroot; n++) {
+      RET(traj_cg(T, n, iters));
+      RET(traj_force(T, n, f_eps * OM_LAMBDA_MID));
+    }
+    RET(update_gauge_step(T, g_eps, &tr));
+    T.gnorm += tr;
+    if (tr > T.max_gf) T.max_gf = tr;
+    for (int n = 0; n < c->p.Nroot; n++) {
+      RET(traj_cg(T, n, iters));
+      RET(traj_force(T, n, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA)));
+    }
+  }
+  return 0;
+}
+
+static int traj_action(Traj &T, double *parts) {
+  susy_ctx *c = T.c;
+  double a4[4];
+  RET(actions_dev(c, a4));
+  double tot = a4[0] + (c->p.bmass > IMAG_TOL ? a4[1] : 0.0) +
+               (c->p.kappa_u1 > IMAG_TOL ? a4[2] : 0.0);
+  double f = 0;
+  for (int n = 0; n < c->p.Nroot; n++) {
+    double fa;
+    RET(fermion_action_ptr(c, T.gsrc[n], T.psim[n].data(), &fa));
+    f += fa;
+  }
+  tot += f + a4[3];
+  parts[0] = a4[0];
+  parts[1] = c->p.bmass > IMAG_TOL ? a4[1] : 0.0;
+  parts[2] = c->p.kappa_u1 > IMAG_TOL ? a4[2] : 0.0;
+  parts[3] = f;
+  parts[4] = a4[3];
+  parts[5] = tot;
+  return 0;
+}
+
+__global__ void k_neg(double2 *__restrict__ y, const double2 *__restrict__ x, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = make_double2(-x[i].x, -x[i].y);
+}
+
+extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
+  if (!c) return SUSY_E_STATE;
+  Traj T;
+  T.c = c;
+  T.form = form;
+  const int Nr = c->p.Nroot, No = c->p.Norder;
+  // per-trajectory fields are kept in the context across calls
+  std::vector<double2 *> &pool = c->traj_pool;
+  size_t need = (size_t)Nr * (1 + No);
+  while (pool.size() < need) {
+    double2 *p;
+    RET(dalloc(&p, tf_elems(c)));
+    pool.push_back(p);
+  }
+  T.gsrc.resize(Nr);
+  T.psim.resize(Nr);
+  for (int n = 0; n < Nr; n++) {
+    T.gsrc[n] = pool[n * (1 + No)];
+    T.psim[n].assign(pool.begin() + n * (1 + No) + 1, pool.begin() + (n + 1) * (1 + No));
+  }
+  int iters = 0;
+  RET(ranmom(c));
+  for (int n = 0; n < Nr; n++) RET(grsource(c, T.gsrc[n], T.psim[n].data(), form, &iters));
+  for (int n = 0; n < Nr; n++) RET(traj_cg(T, n, &iters));
+  double p0[6], p1[6];
+  RET(traj_action(T, p0));
+  const size_t mat5 = (size_t)NUMLINK * NN * c->V;
+  HIPCHK(hipMemcpyAsync(c->d_old_links, c->d_links, sizeof(double2) * mat5,
+                        hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->d_old_mom, c->d_mom, sizeof(double2) * mat5,
+                        hipMemcpyDeviceToDevice, c->s));
+  RET(update_step(T, &iters));
+  RET(traj_action(T, p1));
+  double change = p1[5] - p0[5];
+  if (fabs(change) > 1e20 || change != change) change = 1.0e20;
+  double xr = prn_next(&c->node_prn);
+  int accept = 1;
+  if (exp(-change) < xr) {
+    accept = 0;
+    if (c->p.traj_length > 0.0) {
+      HIPCHK(hipMemcpyAsync(c->d_links, c->d_old_links, sizeof(double2) * mat5,
+                            hipMemcpyDeviceToDevice, c->s));
+      hipLaunchKernelGGL(k_neg, dim3((mat5 + 255) / 256), dim3(256), 0, c->s, c->d_mom,
+                         c->d_old_mom, mat5);
+      HIPCHK(hipGetLastError());
+      RET(fields_recompute(c));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (stats) {
+    stats[0] = p0[5]; stats[1] = p1[5]; stats[2] = change; stats[3] = accept;
+    stats[4] = iters;
+    stats[5] = T.gnorm / (2.0 * c->p.nstep); stats[6] = T.max_gf;
+    for (int n = 0; n < 2; n++) {
+      stats[7 + 2 * n] = n < Nr ? T.fnorm[n] / (2.0 * c->p.nstep) : 0;
+      stats[8 + 2 * n] = n < Nr ? T.max_ff[n] : 0;
+    }
+    for (int k = 0; k < 6; k++) { stats[11 + k] = p0[k]; stats[17 + k] = p1[k]; }
+  }
+  return accept;
+}
+
+// =====================================================================
+// Drop-in boundary: the reference's symbols, backed by one context built
+// from the bound geometry and the caller's globals.
+static struct {
+  susy_ctx *c = nullptr;
+  susy_params p;
+  bool have_lattice = false, have_couplings = false;
+  char *sites = nullptr;
+  size_t site_bytes = 0, link_off = 0, mom_off = 0;
+  int *Norder = nullptr, *total_iters = nullptr;
+  Real **shift = nullptr, **amp4 = nullptr;
+  std::vector<double2 *> sol;   // device fields for multi-field calls
+  std::vector<double> host;     // staging for site-array gathers
+} g_drop;
+
+static int drop_ctx() {
+  if (g_drop.c) return 0;
+  if (!g_drop.have_lattice || !g_drop.have_couplings) {
+    set_error("susy_bind_lattice and susy_bind_couplings must be called first");
+    return SUSY_E_STATE;
+  }
+  return susy_create(&g_drop.p, &g_drop.c);
+}
+
+extern "C" int susy_bind_lattice(int nx, int ny, int nz, int nt, int PBC) {
+  if (g_drop.c) {
+    susy_destroy(g_drop.c);
+    g_drop.c = nullptr;
+  }
+  susy_params &p = g_drop.p;
+  memset(&p, 0, sizeof(p));
+  p.nx = nx; p.ny = ny; p.nz = nz; p.nt = nt; p.PBC = PBC;
+  p.iseed = 0; p.Nroot = 1; p.Norder = 15; p.lambda = 1.0;
+  p.niter = 1000; p.error_per_site = 1e-5; p.traj_length = 1; p.nstep = 1;
+  p.nstep_gauge = 1;
+  g_drop.have_lattice = true;
+  return 0;
+}
+
+extern "C" int susy_bind_couplings(double lambda, double kappa_u1, double bmass,
+                                   double fmass, double G) {
+  if (!g_drop.have_lattice) {
+    set_error("susy_bind_lattice must be called first");
+    return SUSY_E_STATE;
+  }
+  if (g_drop.c) {
+    susy_destroy(g_drop.c);
+    g_drop.c = nullptr;
+  }
+  g_drop.p.lambda = lambda; g_drop.p.kappa_u1 = kappa_u1; g_drop.p.bmass = bmass;
+  g_drop.p.fmass = fmass; g_drop.p.G = G;
+  g_drop.have_couplings = true;
+  return drop_ctx();
+}
+
+extern "C" int susy_bind_sites(void *lattice, size_t site_bytes, size_t link_offset,
+                               size_t mom_offset) {
+  if (!lattice || site_bytes < sizeof(susy_matrix)) return SUSY_E_ARG;
+  g_drop.sites = (char *)lattice;
+  g_drop.site_bytes = site_bytes;
+  g_drop.link_off = link_offset;
+  g_drop.mom_off = mom_offset;
+  return 0;
+}
+
+extern "C" int susy_bind_rhmc(int *Norder, Real **shift, Real **amp4, int *total_iters) {
+  g_drop.Norder = Norder;
+  g_drop.shift = shift;
+  g_drop.amp4 = amp4;
+  g_drop.total_iters = total_iters;
+  return 0;
+}
+
+static void gather_sites(size_t off, double *out) {
+  const size_t per = sizeof(susy_matrix) * NUMLINK;
+  for (int i = 0; i < g_drop.c->V; i++)
+    memcpy(out + (size_t)i * per / sizeof(double), g_drop.sites + i * g_drop.site_bytes + off,
+           per);
+}
+static void scatter_sites(size_t off, const double *in) {
+  const size_t per = sizeof(susy_matrix) * NUMLINK;
+  for (int i = 0; i < g_drop.c->V; i++)
+    memcpy(g_drop.sites + i * g_drop.site_bytes + off, in + (size_t)i * per / sizeof(double),
+           per);
+}
+
+extern "C" int susy_links_changed(void) {
+  RET(drop_ctx());
+  if (!g_drop.sites) {
+    set_error("susy_bind_sites must be called first");
+    return SUSY_E_STATE;
+  }
+  g_drop.host.resize((size_t)g_drop.c->V * NUMLINK * NN * 2);
+  gather_sites(g_drop.link_off, g_drop.host.data());
+  return susy_set_links(g_drop.c, g_drop.host.data());
+}
+
+static void fatal(const char *what) {
+  fprintf(stderr, "susy_hip: %s failed: %s\n", what, susy_last_error());
+  fflush(stderr);
+  exit(1);
+}
+
+static int drop_sol(int n) {
+  while ((int)g_drop.sol.size() < n) {
+    double2 *p;
+    RET(dalloc(&p, tf_elems(g_drop.c)));
+    g_drop.sol.push_back(p);
+  }
+  return 0;
+}
+
+extern "C" void fermion_op(Twist_Fermion *src, Twist_Fermion *dest, int sign) {
+  if (sign != 1 && sign != -1) {
+    fprintf(stderr, "Error: incorrect sign in fermion_op: %d\n", sign);
+    exit(1);
+  }
+  susy_ctx *c;
+  if (drop_ctx() || drop_sol(2)) fatal("fermion_op");
+  c = g_drop.c;
+  if (upload_aos(c, (const double *)src, g_drop.sol[0], NTF) ||
+      dslash64(c, g_drop.sol[0], g_drop.sol[1], sign, nullptr, 0.0, nullptr, nullptr) ||
+      download_aos(c, g_drop.sol[1], (double *)dest, NTF))
+    fatal("fermion_op");
+}
+
+extern "C" void DSq(Twist_Fermion *src, Twist_Fermion *dest) {
+  susy_ctx *c;
+  if (drop_ctx() || drop_sol(2)) fatal("DSq");
+  c = g_drop.c;
+  if (upload_aos(c, (const double *)src, g_drop.sol[0], NTF) ||
+      dsq64(c, g_drop.sol[0], g_drop.sol[1]) ||
+      download_aos(c, g_drop.sol[1], (double *)dest, NTF))
+    fatal("DSq");
+}
+
+extern "C" int congrad_multi(Twist_Fermion *src, Twist_Fermion **psim, int MaxCG,
+                             Real errormin, Real *size_r) {
+  if (drop_ctx()) fatal("congrad_multi");
+  if (!g_drop.Norder || !g_drop.shift) {
+    set_error("susy_bind_rhmc must be called first");
+    fatal("congrad_multi");
+  }
+  susy_ctx *c = g_drop.c;
+  int N = *g_drop.Norder;
+  if (drop_sol(N + 1)) fatal("congrad_multi");
+  if (upload_aos(c, (const double *)src, g_drop.sol[N], NTF)) fatal("congrad_multi");
+  double sr = 0;
+  int before = c->total_iters;
+  int it = cg_multi(c, g_drop.sol[N], g_drop.sol.data(), N, *g_drop.shift, MaxCG, errormin,
+                    &sr, SUSY_FORM_FP64);
+  if (it < 0) fatal("congrad_multi");
+  for (int j = 0; j < N; j++)
+    if (download_aos(c, g_drop.sol[j], (double *)psim[j], NTF)) fatal("congrad_multi");
+  if (g_drop.total_iters) *g_drop.total_iters += c->total_iters - before;
+  *size_r = sr;
+  return it;
+}
+
+extern "C" double fermion_force(Real eps, Twist_Fermion *src, Twist_Fermion **sol) {
+  (void)src;
+  if (drop_ctx()) fatal("fermion_force");
+  if (!g_drop.Norder || !g_drop.amp4 || !g_drop.sites) {
+    set_error("susy_bind_rhmc and susy_bind_sites must be called first");
+    fatal("fermion_force");
+  }
+  susy_ctx *c = g_drop.c;
+  int N = *g_drop.Norder;
+  if (drop_sol(N)) fatal("fermion_force");
+  for (int j = 0; j < N; j++)
+    if (upload_aos(c, (const double *)sol[j], g_drop.sol[j], NTF)) fatal("fermion_force");
+  g_drop.host.resize((size_t)c->V * NUMLINK * NN * 2);
+  gather_sites(g_drop.mom_off, g_drop.host.data());
+  if (susy_set_mom(c, g_drop.host.data())) fatal("fermion_force");
+  double nm = 0;
+  if (fermion_force_dev(c, eps, g_drop.sol.data(), N, *g_drop.amp4, &nm)) fatal("fermion_force");
+  if (susy_get_mom(c, g_drop.host.data())) fatal("fermion_force");
+  scatter_sites(g_drop.mom_off, g_drop.host.data());
+  return nm;
+}
+
+extern "C" double gauge_force(Real eps) {
+  if (drop_ctx()) fatal("gauge_force");
+  if (!g_drop.sites) {
+    set_error("susy_bind_sites must be called first");
+    fatal("gauge_force");
+  }
+  susy_ctx *c = g_drop.c;
+  g_drop.host.resize((size_t)c->V * NUMLINK * NN * 2);
+  gather_sites(g_drop.mom_off, g_drop.host.data());
+  if (susy_set_mom(c, g_drop.host.data())) fatal("gauge_force");
+  double nm = 0;
+  if (gauge_force_dev(c, eps, &nm)) fatal("gauge_force");
+  if (susy_get_mom(c, g_drop.host.data())) fatal("gauge_force");
+  scatter_sites(g_drop.mom_off, g_drop.host.data());
+  return nm;
+}
