@@ -1,0 +1,189 @@
+"""Benchmark of the RHMC hot path on one MI355X.
+
+Workload (the production configuration of the metric): U(4) on a 16^3 x 32
+lattice, lambda = 2.0, bmass = 0.4, G = 0.05, kappa_u1 = 0, fmass = 0,
+Nroot = 2, Norder = 15, antiperiodic temporal fermion BC, trajectory length 1,
+nstep = 10 fermion and nstep_gauge = 10 gauge steps, CG stopping at
+|r|^2 <= 1e-10 |src|^2.  Synthetic data: cold start (unit links), momenta and
+pseudofermions from the reference's generators seeded with 1234.
+
+One step = one full RHMC trajectory (heatbath, multi-shift solves, 2-level
+Omelyan MD with fermion and gauge forces, Metropolis test).  The headline
+form is FP32 operator + multi-shift CG with FP64 reliable updates; the plain
+FP64 form runs from the same start on the same momenta and pseudofermions
+for the speed-up and the error of dH.
+
+Prints one JSON line (rank 0).  Only one GPU is supported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = (16, 16, 16, 32)
+NCOL = 4
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(nstep, nstep_gauge):
+    return dict(dims=DIMS, PBC=-1, iseed=1234, Nroot=2, Norder=15, lam=2.0, kappa_u1=0.0,
+                bmass=0.4, fmass=0.0, G=0.05, niter=5000, error_per_site=1e-5,
+                traj_length=1.0, nstep=nstep, nstep_gauge=nstep_gauge)
+
+
+def dslash_bytes(V, nbytes_complex):
+    """algorithmic bytes of one fermion_op: 16 source + 16 destination + 5
+    link matrices of NCOL^2 complex numbers per site"""
+    return 37 * NCOL * NCOL * nbytes_complex * V
+
+
+def run_form(lat, form, steps, start_links, prof_every):
+    """`steps` trajectories from start_links with freshly seeded generators"""
+    import susy_amd  # noqa: F401
+    lat.set_links(start_links)
+    lat.reseed(1234)
+    lat.prof_enable(prof_every)
+    t0 = time.perf_counter()
+    stats = [lat.update(form) for _ in range(steps)]
+    t1 = time.perf_counter()
+    prec = 1 if form == 1 else 0
+    avg_ms, launches, sampled = lat.prof_read(prec)
+    lat.prof_enable(0)
+    return stats, t1 - t0, (avg_ms, launches, sampled)
+
+
+def cpu_baseline(fp64_stats, nstep, seconds_budget=30.0):
+    """The C oracle (a restatement of the reference, single core) timed on a
+    sub-volume of the same workload: per-site costs of one DSq, one
+    15-shift CG iteration and one fermion-force pole on U(4) 8^3 x 16, scaled
+    to 16^3 x 32 and to the operation counts of the plain FP64 GPU trajectory."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    from oracle_lib import Oracle, random_links, random_tf
+    sub = (8, 8, 8, 16)
+    kw = workload(nstep, 10)
+    kw["dims"] = sub
+    O = Oracle(NCOL, **kw)
+    O.set_links(random_links(O.link_shape(), 1, 0.1))
+    src = random_tf(O.tf_shape(), 2)
+    shifts = O.rhmc()[16:31]
+    O.congrad_multi(src, shifts, 1, 1e-30)          # first touch of the buffers
+    t = time.perf_counter()
+    O.congrad_multi(src, shifts, 3, 1e-30)
+    t_cg1 = (time.perf_counter() - t) / 3
+    _, _, ps = O.congrad_multi(src, shifts[:1], 1, 1e-30)
+    t = time.perf_counter()
+    O.fermion_force(0.01, ps[:1], O.rhmc()[1:2])
+    t_pole = time.perf_counter() - t
+    scale = np.prod(DIMS) / np.prod(sub)
+    iters = np.mean([s[4] for s in fp64_stats])
+    nforce = 2 * (2 * nstep + 1)                  # Nroot (2 nstep + 1) forces
+    t_traj = scale * (iters * t_cg1 + nforce * 15 * t_pole)
+    acc = np.mean([s[3] for s in fp64_stats])
+    return {"value": 3600.0 / t_traj * acc, "unit": "accepted traj/hour", "cores": 1,
+            "kind": "port",
+            "sample": (f"C oracle, 1 core, U(4) 8^3x16 sub-volume: {t_cg1:.2f} s per 15-shift "
+                       f"CG iteration, {t_pole:.2f} s per force pole; scaled x{scale:.0f} to "
+                       f"16^3x32 and to the FP64 GPU trajectory's {iters:.0f} CG iterations "
+                       f"and {nforce}x15 force poles; acceptance {acc:.2f} from the GPU run "
+                       f"(est. {t_traj:.0f} s per trajectory)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed FP32-form trajectories")
+    ap.add_argument("--warmup", type=int, default=0, help="untimed FP32-form trajectories")
+    ap.add_argument("--fp64-steps", type=int, default=1, help="timed plain FP64 trajectories")
+    ap.add_argument("--nstep", type=int, default=10)
+    ap.add_argument("--nstep-gauge", type=int, default=10)
+    ap.add_argument("--prof-every", type=int, default=8,
+                    help="time every n-th operator kernel launch with HIP events")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not an implementation",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import susy_amd
+    lat = susy_amd.Lattice(NCOL, **workload(args.nstep, args.nstep_gauge))
+    V = lat.V
+    start = lat.get_links()
+    # warm-up trajectories move the start away from the cold configuration
+    for _ in range(args.warmup):
+        lat.update(susy_amd.FORM_FP32)
+    if args.warmup:
+        start = lat.get_links()
+
+    # isolated operator timing (HIP events, 20 launches each)
+    h = lat.tf_alloc(2)
+    rng = np.random.default_rng(5)
+    lat.tf_upload(h, (rng.standard_normal(lat.tf_shape())
+                      + 1j * rng.standard_normal(lat.tf_shape())) / np.sqrt(2))
+    iso = {nm: lat.time_fermion_op(h, h + 1, form=f, reps=20)
+           for nm, f in (("fp64", 0), ("fp32", 1))}
+
+    st32, t32, prof32 = run_form(lat, susy_amd.FORM_FP32, args.steps, start, args.prof_every)
+    st64, t64, prof64 = run_form(lat, susy_amd.FORM_FP64, args.fp64_steps, start,
+                                 args.prof_every)
+    ss, st = lat.plaquette()
+
+    def summary(stats, secs):
+        acc = sum(s[3] for s in stats)
+        return {"traj_per_hour": 3600.0 * len(stats) / secs,
+                "accepted_traj_per_hour": 3600.0 * acc / secs,
+                "acceptance": acc / len(stats),
+                "dH": [float(s[2]) for s in stats],
+                "exp_minus_dH": float(np.mean([np.exp(-s[2]) for s in stats])),
+                "cg_iters": [int(s[4]) for s in stats],
+                "s_per_traj": secs / len(stats)}
+
+    f32, f64 = summary(st32, t32), summary(st64, t64)
+    # first trajectories start from the same links, momenta and pseudofermions
+    dh_err = abs(st32[0][2] - st64[0][2])
+    bytes32 = dslash_bytes(V, 8)
+    avg_ms = prof32[0]
+    achieved = bytes32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    out = {
+        "metric": "accepted traj/hour, U(4) 16^3x32: FP32+surrogate vs plain FP64; Dslash GB/s",
+        "value": f32["accepted_traj_per_hour"],
+        "unit": "accepted traj/hour",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "4d_Q16 RHMC U(4) 16^3x32 (lambda 2.0, bmass 0.4, G 0.05, "
+                               "Nroot 2, Norder 15), FP32 form; cold start",
+                   "lattice": "16x16x16x32", "nstep": args.nstep,
+                   "nstep_gauge": args.nstep_gauge, "form": "FP32 + FP64 reliable updates",
+                   "surrogate": "not used (learned force not built yet)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_dslash<float> (fermion_op, FP32)",
+                     "launches": prof32[1], "sampled": prof32[2], "avg_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes32},
+        "forms": {"fp32": f32, "fp64": f64,
+                  "speedup_fp32_vs_fp64": f32["traj_per_hour"] / f64["traj_per_hour"],
+                  "dH_abs_diff_first_traj": dh_err},
+        "dslash": {"fp64_ms": iso["fp64"],
+                   "fp64_GBs": dslash_bytes(V, 16) / iso["fp64"] / 1e6,
+                   "fp32_ms": iso["fp32"], "fp32_GBs": bytes32 / iso["fp32"] / 1e6,
+                   "fp64_in_trajectory_ms": prof64[0]},
+        "plaquette_end": [ss, st],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(st64, args.nstep)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

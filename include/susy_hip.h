@@ -173,6 +173,8 @@ int susy_plaquette_dev(susy_ctx *c, double *ss, double *st);
 // src = (D^dag D)^(1/8 Nroot) g with the solutions of its solve in
 // h_psim .. h_psim+Norder-1.
 int susy_ranmom(susy_ctx *c);
+// Restart all generators from `seed` (as at start-up).
+int susy_reseed(susy_ctx *c, int seed);
 int susy_grsource_dev(susy_ctx *c, int h_src, int h_psim, int form, int *iters);
 
 // One RHMC trajectory (update_o.c:127-282) in the given form.
@@ -186,6 +188,13 @@ int susy_update(susy_ctx *c, int form, double *stats);
 // library's own stream).  Returns average milliseconds per call.
 int susy_time_fermion_op(susy_ctx *c, int h_src, int h_dst, int form,
                          int reps, double *ms_per_call);
+// Sampled HIP-event timing of every operator-kernel launch the context makes
+// (in CG, forces, trajectories): every `every`-th launch is timed (0 = off).
+// susy_prof_read returns the average sampled duration in ms and the number
+// of launches for prec 0 (FP64 kernel) or 1 (FP32 kernel).
+int susy_prof_enable(susy_ctx *c, int every);
+int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
+                   long *sampled);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,7 @@ EXPORTS = [
     "susy_gauge_force_dev", "susy_update_u_dev", "susy_actions_dev",
     "susy_fermion_action_dev", "susy_plaquette_dev", "susy_update", "susy_time_fermion_op",
     "susy_set_cg_options", "susy_ranmom", "susy_grsource_dev", "susy_validate_tables",
+    "susy_prof_enable", "susy_prof_read", "susy_reseed",
 ]
 
 _libs = {}
@@ -81,6 +82,8 @@ def load(ncol):
         "susy_actions_dev": (i, [vp, dp]), "susy_fermion_action_dev": (i, [vp, i, i, dp]),
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
+        "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]),
+        "susy_prof_read": (i, [vp, i, dp, P(ctypes.c_long), P(ctypes.c_long)]),
         "susy_set_cg_options": (i, [i, i]), "susy_ranmom": (i, [vp]),
         "susy_grsource_dev": (i, [vp, i, i, i, ip]),
         "susy_bind_lattice": (i, [i, i, i, i, i]),
@@ -248,6 +251,19 @@ class Lattice:
         ms = ctypes.c_double()
         self._check(self.L.susy_time_fermion_op(self.h, hs, hd, form, reps, ctypes.byref(ms)))
         return ms.value
+
+    def prof_enable(self, every):
+        self._check(self.L.susy_prof_enable(self.h, every))
+
+    def prof_read(self, prec):
+        """(average ms of sampled operator-kernel launches, launches, sampled)"""
+        ms, n, k = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        self._check(self.L.susy_prof_read(self.h, prec, ctypes.byref(ms), ctypes.byref(n),
+                                          ctypes.byref(k)))
+        return ms.value, n.value, k.value
+
+    def reseed(self, seed):
+        self._check(self.L.susy_reseed(self.h, seed))
 
     def ranmom(self):
         self._check(self.L.susy_ranmom(self.h))

@@ -57,6 +57,8 @@ extern "C" int susy_validate_tables(void) {
 }
 
 static void free_ctx(susy_ctx *c) {
+  for (int p = 0; p < 2; p++)
+    for (hipEvent_t e : c->prof_ev[p]) (void)hipEventDestroy(e);
   void *ptrs[] = {c->d_nbr, c->d_t, c->d_links, c->d_mom, c->d_old_links, c->d_old_mom,
                   c->d_uinv, c->d_det, c->d_pdet, c->d_zw, c->d_dmu, c->d_fmunu,
                   c->d_force, c->f_links, c->f_uinv, c->f_pdet, c->d_aux, c->f_aux,
@@ -138,7 +140,7 @@ static int ctx_setup(susy_ctx *c) {
   RET(dalloc(&c->f_aux, (size_t)6 * V));
   RET(dalloc(&c->d_tmp, tfn));
   RET(dalloc(&c->d_boundary, tfn));
-  int chunks = (V + 127) / 128;
+  int chunks = (V + DS_BLOCK - 1) / DS_BLOCK;
   int ds_blocks = NTF * ((chunks + 7) / 8 * 8);
   c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
   RET(dalloc(&c->d_part, (size_t)c->npart * 2));
@@ -581,6 +583,20 @@ static int ranmom(susy_ctx *c) {
   std::vector<double> h((size_t)c->V * NUMLINK * NN * 2);
   heatbath_fill(c, h.data(), NUMLINK, false);
   return upload_aos(c, h.data(), c->d_mom, NUMLINK);
+}
+
+// Restart the site and node generators from a seed, as at start-up
+// (make_lattice.c:48, setup.c:246).
+extern "C" int susy_reseed(susy_ctx *c, int seed) {
+  if (!c) return SUSY_E_STATE;
+  const susy_params &p = c->p;
+  for (int i = 0; i < c->V; i++) {
+    int lex = c->geo.x[i] + p.nx * (c->geo.y[i] + p.ny * (c->geo.z[i] + p.nz * c->geo.t[i]));
+    prn_init(&c->site_prn[i], seed, lex);
+  }
+  prn_init(&c->node_prn, seed, c->V);
+  c->p.iseed = seed;
+  return 0;
 }
 
 extern "C" int susy_ranmom(susy_ctx *c) {

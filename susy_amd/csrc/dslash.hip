@@ -14,9 +14,17 @@
 // accumulates in registers, the right operand of each product is held in
 // registers while the left operand streams element by element.
 //
+// Memory access goes through raw buffer loads: each field has a 128-bit buffer
+// descriptor in SGPRs, a lane supplies only the 32-bit byte offset of its
+// site, and the matrix-element offset is a uniform SGPR offset computed on
+// the scalar unit.  This keeps address arithmetic out of the VGPR budget, and
+// the descriptor's record count bounds every access.
+//
 // Block mapping is XCD-aware: consecutive hardware block ids are dealt to the
-// 8 XCDs round-robin, so the 16 component-blocks of one site chunk are given
-// ids that land on the same XCD and share its L2.
+// 8 XCDs round-robin.  The ids are remapped so that each XCD walks one
+// contiguous slab of site chunks, all 16 component-blocks of a chunk back to
+// back: neighbouring chunks (the z and t neighbours of a site) then share the
+// same L2.
 #include "susy_internal.h"
 
 __constant__ PlaqTerms c_pt;
@@ -26,7 +34,10 @@ int dslash_init_tables(const PlaqTerms &pt) {
   return 0;
 }
 
-#define DS_BLOCK 128
+// minimum wavefronts per SIMD the register allocator must allow
+#ifndef DS_WAVES
+#define DS_WAVES 2
+#endif
 
 template <typename C, typename R>
 __device__ __forceinline__ void cfma(C &acc, R ar, R ai, C b) {
@@ -36,32 +47,66 @@ __device__ __forceinline__ void cfma(C &acc, R ar, R ai, C b) {
   acc.y = fma(ai, b.x, acc.y);
 }
 
-// acc += s * op(A) * op(B); A, B point at element (0,0) of a matrix at the
-// wanted site; elements are V apart.  TA/TB select the dagger.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void *p, uint32_t bytes) {
+  // dword3 0x00020000: raw (untyped) access on the gfx9 family
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+template <typename C> __device__ __forceinline__ C bload(Rsrc r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ double2 bload<double2>(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float2 bload<float2>(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ int bload_i32(Rsrc r, uint32_t voff, uint32_t soff) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+template <typename C> __device__ __forceinline__ void bstore(C v, Rsrc r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ void bstore<double2>(double2 v, Rsrc r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<float2>(float2 v, Rsrc r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, voff, soff, 0);
+}
+
+// A matrix operand: field descriptor, lane byte offset of the site, uniform
+// byte offset of element (0,0); element (i,j) is (i*NCOL+j) planes further.
+struct Opnd {
+  Rsrc r;
+  uint32_t v, s;
+};
+
+// acc += s * op(A) * op(B).  TA/TB select the dagger.  B is streamed one row
+// at a time (row j of op(B) meets column j of op(A)), so every operand
+// element is loaded once and only the accumulator plus one row stay live.
 template <typename R, bool TA, bool TB>
-__device__ __forceinline__ void mmacc(typename CT<R>::t (&acc)[NCOL][NCOL], R s,
-                                      const typename CT<R>::t *__restrict__ A,
-                                      const typename CT<R>::t *__restrict__ B,
-                                      int V) {
+__device__ __forceinline__ void mmacc(typename CT<R>::t (&acc)[NCOL][NCOL], R s, Opnd A,
+                                      Opnd B, uint32_t plane) {
   typedef typename CT<R>::t C;
-  C b[NCOL][NCOL];
 #pragma unroll
-  for (int i = 0; i < NCOL; i++)
+  for (int j = 0; j < NCOL; j++) {
+    C brow[NCOL];
 #pragma unroll
-    for (int j = 0; j < NCOL; j++) {
-      C v = TB ? B[(j * NCOL + i) * V] : B[(i * NCOL + j) * V];
+    for (int k = 0; k < NCOL; k++) {
+      C v = bload<C>(B.r, B.v, B.s + (uint32_t)(TB ? k * NCOL + j : j * NCOL + k) * plane);
       if (TB) v.y = -v.y;
-      b[i][j] = v;
+      brow[k] = v;
     }
 #pragma unroll
-  for (int i = 0; i < NCOL; i++)
-#pragma unroll
-    for (int j = 0; j < NCOL; j++) {
-      C a = TA ? A[(j * NCOL + i) * V] : A[(i * NCOL + j) * V];
+    for (int i = 0; i < NCOL; i++) {
+      C a = bload<C>(A.r, A.v, A.s + (uint32_t)(TA ? j * NCOL + i : i * NCOL + j) * plane);
       R ar = s * a.x, ai = TA ? -s * a.y : s * a.y;
 #pragma unroll
-      for (int k = 0; k < NCOL; k++) cfma(acc[i][k], ar, ai, b[j][k]);
+      for (int k = 0; k < NCOL; k++) cfma(acc[i][k], ar, ai, brow[k]);
     }
+  }
 }
 
 template <typename R>
@@ -111,62 +156,71 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
 template <typename R, bool DAG, int EPI>
-__global__ void __launch_bounds__(DS_BLOCK)
+__global__ void __launch_bounds__(DS_BLOCK) __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
          const int *__restrict__ nbr, const int *__restrict__ tco, int V, int nt,
-         R pbc, int doG, R G, int nchunks, const typename CT<R>::t *__restrict__ p0,
+         R pbc, int doG, R G, int nchunks, int nchunks_pad,
+         const typename CT<R>::t *__restrict__ p0,
          R shift, double *__restrict__ partial) {
   typedef typename CT<R>::t C;
   const int L = blockIdx.x;
-  const int slot = L >> 3;
-  const int comp = slot % NTF;
-  const int chunk = (slot / NTF) * 8 + (L & 7);
+  const int q = L >> 3;                       // position in this XCD's queue
+  const int comp = q % NTF;
+  const int chunk = (L & 7) * (nchunks_pad >> 3) + q / NTF;
   const int s = chunk * DS_BLOCK + threadIdx.x;
   const bool live = chunk < nchunks && s < V;
 
+  const uint32_t cs = sizeof(C);
+  const uint32_t pl = (uint32_t)V * cs;     // one element plane
+  const uint32_t Mb = NN * pl;              // one matrix field
+  const Rsrc rS = make_rsrc(src, NTF * Mb), rU = make_rsrc(U, NUMLINK * Mb);
+  const Rsrc rN = make_rsrc(nbr, (uint32_t)NGATHER * V * 4u);
+  auto nb = [&](int g) { return bload_i32(rN, (uint32_t)s * 4u, (uint32_t)g * V * 4u); };
+  auto S = [&](int k, int x) { return Opnd{rS, (uint32_t)x * cs, (uint32_t)k * Mb}; };
+  auto Ul = [&](int a, int x) { return Opnd{rU, (uint32_t)x * cs, (uint32_t)a * Mb}; };
+
   double dot = 0.0;
   if (live) {
-    const size_t M = (size_t)NN * V;
-    const C *S_site = src + s * 0;
     C acc[NCOL][NCOL];
 #pragma unroll
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
       for (int k = 0; k < NCOL; k++) acc[i][k] = mkc<C>(0, 0);
     const int t = tco[s];
-    (void)S_site;
 
     if (comp == 0) {
       // ---- site output: 1/2 sum_a [psi_a(x) Ubar_a(x) - bc(-a) Ubar_a(x-a) psi_a(x-a)]
 #pragma unroll 1
       for (int a = 0; a < NUMLINK; a++) {
-        int xm = nbr[g_bwd(a) * V + s];
-        mmacc<R, DAG, true>(acc, (R)0.5, src + (1 + a) * M + s, U + a * M + s, V);
-        mmacc<R, true, DAG>(acc, (R)-0.5 * bcsign<R>(t, -link_tau(a), nt, pbc),
-                            U + a * M + xm, src + (1 + a) * M + xm, V);
+        const int xm = nb(g_bwd(a));
+        mmacc<R, DAG, true>(acc, (R)0.5, S(1 + a, s), Ul(a, s), pl);
+        mmacc<R, true, DAG>(acc, (R)-0.5 * bcsign<R>(t, -link_tau(a), nt, pbc), Ul(a, xm),
+                            S(1 + a, xm), pl);
       }
       if (doG) {
         // detLtoS: -(i G C2 sqrt(N)/2) sum_{a!=b} plaqdet_ab [T_b(x) + bc(b) T_a(x+b)]
         //          * Lambda_{DIMF-1},  Lambda_{DIMF-1} = i I / sqrt(N)
+        const Rsrc rA = make_rsrc(aux, 6 * pl), rP = make_rsrc(pdet, NUMLINK * NUMLINK * pl);
         C sum = mkc<C>(0, 0);
 #pragma unroll 1
-        for (int a = 0; a < NUMLINK; a++) {
+        for (int b = 0; b < NUMLINK; b++) {
+          const int xb = nb(g_fwd(b));
+          const C tb = bload<C>(rA, (uint32_t)s * cs, (1 + b) * pl);
+          const R sg = bcsign<R>(t, link_tau(b), nt, pbc);
 #pragma unroll 1
-          for (int b = 0; b < NUMLINK; b++) {
+          for (int a = 0; a < NUMLINK; a++) {
             if (a == b) continue;
-            int xb = nbr[g_fwd(b) * V + s];
-            C tb = aux[(1 + b) * V + s], ta = aux[(1 + a) * V + xb];
-            R sg = bcsign<R>(t, link_tau(b), nt, pbc);
-            C t2 = mkc<C>(tb.x + sg * ta.x, tb.y + sg * ta.y);
-            C pd = pdet[(a * NUMLINK + b) * V + s];
+            const C ta = bload<C>(rA, (uint32_t)xb * cs, (1 + a) * pl);
+            const C t2 = mkc<C>(tb.x + sg * ta.x, tb.y + sg * ta.y);
+            const C pd = bload<C>(rP, (uint32_t)s * cs, (a * NUMLINK + b) * pl);
             sum.x += pd.x * t2.x - pd.y * t2.y;
             sum.y += pd.x * t2.y + pd.y * t2.x;
           }
         }
         // trd = i lg sum; acc -= trd * (i/sqrt N) I = (lg/sqrt N) sum I
-        R f = (R)(0.5 * C2) * G;  // lg / sqrt(N) = G C2 / 2
+        const R f = (R)(0.5 * C2) * G;  // lg / sqrt(N) = G C2 / 2
 #pragma unroll
         for (int i = 0; i < NCOL; i++) {
           acc[i][i].x += f * sum.x;
@@ -176,46 +230,47 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
     } else if (comp <= NUMLINK) {
       // ---- link output b
       const int b = comp - 1;
-      const int xb = nbr[g_fwd(b) * V + s];
+      const int xb = nb(g_fwd(b));
 #pragma unroll 1
       for (int a = 0; a < NUMLINK; a++) {
         if (a == b) continue;
         // Dminus: U_a(x+b) chi_ab(x) - bc(-a) chi_ab(x-a) U_a(x-a)
-        const int xm = nbr[g_bwd(a) * V + s];
+        const int xm = nb(g_bwd(a));
         const R sg = a < b ? (R)1 : (R)-1;   // chi_ab = sg * stored chi_{min,max}
-        const C *chi = src + (6 + plaq_idx(a, b)) * M;
-        mmacc<R, false, DAG>(acc, sg, U + a * M + xb, chi + s, V);
-        mmacc<R, DAG, false>(acc, -sg * bcsign<R>(t, -link_tau(a), nt, pbc), chi + xm,
-                             U + a * M + xm, V);
+        const int pc = 6 + plaq_idx(a, b);
+        mmacc<R, false, DAG>(acc, sg, Ul(a, xb), S(pc, s), pl);
+        mmacc<R, DAG, false>(acc, -sg * bcsign<R>(t, -link_tau(a), nt, pbc), S(pc, xm),
+                             Ul(a, xm), pl);
       }
       // DbplusStoL: 1/2 [bc(b) eta(x+b) Ubar_b(x) - Ubar_b(x) eta(x)]
-      mmacc<R, DAG, true>(acc, (R)0.5 * bcsign<R>(t, link_tau(b), nt, pbc), src + xb,
-                          U + b * M + s, V);
-      mmacc<R, true, DAG>(acc, (R)-0.5, U + b * M + s, src + s, V);
+      mmacc<R, DAG, true>(acc, (R)0.5 * bcsign<R>(t, link_tau(b), nt, pbc), S(0, xb),
+                          Ul(b, s), pl);
+      mmacc<R, true, DAG>(acc, (R)-0.5, Ul(b, s), S(0, s), pl);
       if (doG) {
         // detStoL: -G C2/2 U_b^-1(x) sum_{c!=b} [Tr eta(x) plaqdet_cb(x)
         //                                        + bc(-c) Tr eta(x-c) plaqdet_bc(x-c)]
+        const Rsrc rA = make_rsrc(aux, 6 * pl), rP = make_rsrc(pdet, NUMLINK * NUMLINK * pl);
+        const Rsrc rI = make_rsrc(Uinv, NUMLINK * Mb);
         C sum = mkc<C>(0, 0);
-        const C te = aux[s];
+        const C te = bload<C>(rA, (uint32_t)s * cs, 0);
 #pragma unroll 1
         for (int c = 0; c < NUMLINK; c++) {
           if (c == b) continue;
-          const int xm = nbr[g_bwd(c) * V + s];
-          C p1 = pdet[(c * NUMLINK + b) * V + s];
-          C p2 = pdet[(b * NUMLINK + c) * V + xm];
-          C tm = aux[xm];
-          R sg = bcsign<R>(t, -link_tau(c), nt, pbc);
+          const int xm = nb(g_bwd(c));
+          const C p1 = bload<C>(rP, (uint32_t)s * cs, (c * NUMLINK + b) * pl);
+          const C p2 = bload<C>(rP, (uint32_t)xm * cs, (b * NUMLINK + c) * pl);
+          const C tm = bload<C>(rA, (uint32_t)xm * cs, 0);
+          const R sg = bcsign<R>(t, -link_tau(c), nt, pbc);
           sum.x += te.x * p1.x - te.y * p1.y + sg * (tm.x * p2.x - tm.y * p2.y);
           sum.y += te.x * p1.y + te.y * p1.x + sg * (tm.x * p2.y + tm.y * p2.x);
         }
-        R f = (R)(-0.5 * C2) * G;
-        R sr = f * sum.x, si = f * sum.y;
-        const C *ui = Uinv + b * M + s;
+        const R f = (R)(-0.5 * C2) * G;
+        const R sr = f * sum.x, si = f * sum.y;
 #pragma unroll
         for (int i = 0; i < NCOL; i++)
 #pragma unroll
           for (int k = 0; k < NCOL; k++) {
-            C u = ui[(i * NCOL + k) * V];
+            const C u = bload<C>(rI, (uint32_t)s * cs, b * Mb + (i * NCOL + k) * pl);
             acc[i][k].x += sr * u.x - si * u.y;
             acc[i][k].y += sr * u.y + si * u.x;
           }
@@ -224,60 +279,55 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       // ---- plaquette output (a,b), a<b
       const int p = comp - 1 - NUMLINK;
       const int a = c_pt.a[p], b = c_pt.b[p];
-      const int xa = nbr[g_fwd(a) * V + s], xb = nbr[g_fwd(b) * V + s];
+      const int xa = nb(g_fwd(a)), xb = nb(g_fwd(b));
       // Dplus: bc(a) U_a(x) psi_b(x+a) - psi_b(x) U_a(x+b)
       //        - bc(b) U_b(x) psi_a(x+b) + psi_a(x) U_b(x+a)
-      mmacc<R, false, DAG>(acc, bcsign<R>(t, link_tau(a), nt, pbc), U + a * M + s,
-                           src + (1 + b) * M + xa, V);
-      mmacc<R, DAG, false>(acc, (R)-1, src + (1 + b) * M + s, U + a * M + xb, V);
-      mmacc<R, false, DAG>(acc, -bcsign<R>(t, link_tau(b), nt, pbc), U + b * M + s,
-                           src + (1 + a) * M + xb, V);
-      mmacc<R, DAG, false>(acc, (R)1, src + (1 + a) * M + s, U + b * M + xa, V);
+      mmacc<R, false, DAG>(acc, bcsign<R>(t, link_tau(a), nt, pbc), Ul(a, s), S(1 + b, xa), pl);
+      mmacc<R, DAG, false>(acc, (R)-1, S(1 + b, s), Ul(a, xb), pl);
+      mmacc<R, false, DAG>(acc, -bcsign<R>(t, link_tau(b), nt, pbc), Ul(b, s), S(1 + a, xb), pl);
+      mmacc<R, DAG, false>(acc, (R)1, S(1 + a, s), Ul(b, xa), pl);
 #pragma unroll 1
       for (int n = 0; n < 3; n++) {
         // DbplusPtoP
         const int c = c_pt.pp_c[p][n];
-        const C *chi = src + (6 + c_pt.pp_de[p][n]) * M;
-        const int x1 = nbr[c_pt.pp_g1[p][n] * V + s];
-        const int x2 = nbr[c_pt.pp_g2[p][n] * V + s];
-        const int xmc = nbr[g_bwd(c) * V + s];
+        const int pc = 6 + c_pt.pp_de[p][n];
+        const int x1 = nb(c_pt.pp_g1[p][n]), x2 = nb(c_pt.pp_g2[p][n]), xmc = nb(g_bwd(c));
         const R tr = (R)c_pt.pp_tr[p][n];
-        mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pp_dt3[p][n], nt, pbc), chi + x2,
-                            U + c * M + x1, V);
-        mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pp_dt2[p][n], nt, pbc),
-                            U + c * M + xmc, chi + x1, V);
+        mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pp_dt3[p][n], nt, pbc), S(pc, x2),
+                            Ul(c, x1), pl);
+        mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pp_dt2[p][n], nt, pbc), Ul(c, xmc),
+                            S(pc, x1), pl);
       }
 #pragma unroll 1
       for (int n = 0; n < 3; n++) {
         // DbminusPtoP
         const int c = c_pt.pm_c[p][n];
-        const C *chi = src + (6 + c_pt.pm_ab[p][n]) * M;
-        const int x2 = nbr[c_pt.pm_g2[p][n] * V + s];
-        const int x1 = nbr[c_pt.pm_g1[p][n] * V + s];
-        const int xmc = nbr[g_bwd(c) * V + s];
+        const int pc = 6 + c_pt.pm_ab[p][n];
+        const int x2 = nb(c_pt.pm_g2[p][n]), x1 = nb(c_pt.pm_g1[p][n]), xmc = nb(g_bwd(c));
         const R tr = (R)c_pt.pm_tr[p][n];
-        mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pm_dt2[p][n], nt, pbc), chi + x2,
-                            U + c * M + x1, V);
-        mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pm_dt3[p][n], nt, pbc),
-                            U + c * M + xmc, chi + x1, V);
+        mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pm_dt2[p][n], nt, pbc), S(pc, x2),
+                            Ul(c, x1), pl);
+        mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pm_dt3[p][n], nt, pbc), Ul(c, xmc),
+                            S(pc, x1), pl);
       }
     }
 
     // ---- store (D^dagger: negated adjoint)
-    C *out = dst + comp * M + s;
-    const C *pp = (EPI == 1) ? p0 + comp * M + s : nullptr;
+    const Rsrc rD = make_rsrc(dst, NTF * Mb);
+    const Rsrc rQ = make_rsrc(EPI == 1 ? p0 : dst, NTF * Mb);
 #pragma unroll
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
       for (int k = 0; k < NCOL; k++) {
         C v = DAG ? mkc<C>(-acc[k][i].x, acc[k][i].y) : acc[i][k];
+        const uint32_t so = comp * Mb + (i * NCOL + k) * pl;
         if (EPI == 1) {
-          C q = pp[(i * NCOL + k) * V];
+          const C q = bload<C>(rQ, (uint32_t)s * cs, so);
           v.x = fma(shift, q.x, v.x);
           v.y = fma(shift, q.y, v.y);
           dot += (double)q.x * (double)v.x + (double)q.y * (double)v.y;
         }
-        out[(i * NCOL + k) * V] = v;
+        bstore<C>(v, rD, (uint32_t)s * cs, so);
       }
   }
   if (EPI == 1) {
@@ -287,9 +337,9 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double t = 0;
-      for (int w = 0; w < DS_BLOCK / 64; w++) t += red[w];
-      partial[L] = t;
+      double tsum = 0;
+      for (int w = 0; w < DS_BLOCK / 64; w++) tsum += red[w];
+      partial[L] = tsum;
     }
   }
 }
@@ -305,6 +355,10 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     return SUSY_E_ARG;
   }
   const int V = c->V;
+  if ((size_t)NTF * NN * V * sizeof(typename CT<R>::t) > 0x7fffffffu) {
+    set_error("lattice too large for 32-bit buffer offsets (V=%d)", V);
+    return SUSY_E_ARG;
+  }
   if (c->doG) {
     int nb = (V + 255) / 256;
     if (sign == 1)
@@ -319,7 +373,14 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \
-                nchunks, p0, shift, partial
+                nchunks, padded, p0, shift, partial
+  const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->prof_every > 0 && (c->prof_launch[pi]++ % c->prof_every) == 0) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->s));
+  }
   if (p0 == nullptr) {
     if (sign == 1)
       hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_BLOCK), 0, c->s, DS_ARGS);
@@ -333,6 +394,43 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   }
 #undef DS_ARGS
   HIPCHK(hipGetLastError());
+  if (e0) {
+    HIPCHK(hipEventRecord(e1, c->s));
+    c->prof_ev[pi].push_back(e0);
+    c->prof_ev[pi].push_back(e1);
+  }
+  return 0;
+}
+
+// Sampled timing of the operator kernel (k_dslash) launches on the context's
+// stream.  prof_every = 0 switches sampling off.
+extern "C" int susy_prof_enable(susy_ctx *c, int every) {
+  if (!c || every < 0) return SUSY_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int p = 0; p < 2; p++) {
+    for (hipEvent_t e : c->prof_ev[p]) (void)hipEventDestroy(e);
+    c->prof_ev[p].clear();
+    c->prof_launch[p] = 0;
+  }
+  c->prof_every = every;
+  return 0;
+}
+// Average duration (ms) of the sampled launches and the total launch count,
+// for FP64 (prec 0) or FP32 (prec 1) operator kernels.
+extern "C" int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
+                              long *sampled) {
+  if (!c || prec < 0 || prec > 1) return SUSY_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  double tot = 0;
+  size_t n = c->prof_ev[prec].size() / 2;
+  for (size_t k = 0; k < n; k++) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->prof_ev[prec][2 * k], c->prof_ev[prec][2 * k + 1]));
+    tot += ms;
+  }
+  *avg_ms = n ? tot / n : 0.0;
+  *launches = c->prof_launch[prec];
+  *sampled = (long)n;
   return 0;
 }
 

@@ -25,6 +25,10 @@
 #define NTERMS 30
 #define NQLINK 28
 #define IMAG_TOL 1.0e-8
+// sites per block of the fused operator kernel
+#ifndef DS_BLOCK
+#define DS_BLOCK 128
+#endif
 #define C2 1.0
 
 // ---------------------------------------------------------------------
@@ -156,6 +160,11 @@ struct susy_ctx {
   int g_iset = 0;
   double g_gset = 0;
   int total_iters = 0;
+  // sampled HIP-event timing of the operator kernel launches (per precision:
+  // index 0 FP64, 1 FP32): every prof_every-th launch is bracketed by events
+  int prof_every = 0;
+  long prof_launch[2] = {0, 0};
+  std::vector<hipEvent_t> prof_ev[2];
 };
 
 void set_error(const char *fmt, ...);
