@@ -48,7 +48,12 @@ def run_form(lat, form, steps, start_links, prof_every):
     lat.reseed(1234)
     lat.prof_enable(prof_every)
     t0 = time.perf_counter()
-    stats = [lat.update(form) for _ in range(steps)]
+    stats = []
+    for k in range(steps):
+        stats.append(lat.update(form))
+        print(f"bench: form {form} trajectory {k}: dH {stats[-1][2]:.4g} accept "
+              f"{int(stats[-1][3])} iters {int(stats[-1][4])} "
+              f"t {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     prec = 1 if form == 1 else 0
     avg_ms, launches, sampled = lat.prof_read(prec)
@@ -115,6 +120,7 @@ def main():
     import numpy as np
 
     import susy_amd
+    susy_amd.load(NCOL).susy_set_verbose(1)   # one progress line per solve (stderr)
     lat = susy_amd.Lattice(NCOL, **workload(args.nstep, args.nstep_gauge))
     V = lat.V
     start = lat.get_links()

@@ -121,6 +121,8 @@ int susy_validate_tables(void);
 // 32 iterations) and, for the FP32 form, iterations between FP64 true
 // residuals.  Process-wide.
 int susy_set_cg_options(int window, int reliable);
+// Progress lines on stderr from susy_update (one per solve) when v != 0.
+int susy_set_verbose(int v);
 int susy_create(const susy_params *p, susy_ctx **out);
 void susy_destroy(susy_ctx *c);
 int susy_volume(const susy_ctx *c);
@@ -153,6 +155,9 @@ int susy_tf_download(susy_ctx *c, int h, Twist_Fermion *dst);
 // Resident operations on handles.
 int susy_fermion_op_dev(susy_ctx *c, int h_src, int h_dst, int sign, int form);
 int susy_dsq_dev(susy_ctx *c, int h_src, int h_dst, int form);
+// Plain copy kernel of one field (reads and writes 16 NCOL^2 V complex
+// numbers once): calibrates hardware memory-traffic counters.
+int susy_stream_copy_dev(susy_ctx *c, int h_src, int h_dst);
 // Multi-shift CG: psim handles h_psim .. h_psim+nshift-1.  Returns iterations
 // (>= 0) or an error code.  form selects the precision form.
 int susy_congrad_multi_dev(susy_ctx *c, int h_src, int h_psim, int nshift,

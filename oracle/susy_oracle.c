@@ -20,10 +20,9 @@
 //     free-field spectrum, gauge covariance, and end-to-end vs the
 //     reference's own regression outputs testsuite/scalar/hmc.U{2,3,4}.ref
 //     (actions to 8 digits, exact CG iteration counts).
-//   * determinant / inverse: U U^-1 = 1 and det multiplicativity; LAPACK
-//     itself is absent from the reference tree, so beyond rounding the
-//     pivoting restatement is "parity pinned end-to-end only" (through the
-//     det action and CHECK values of hmc.U*.ref).
+//   * determinant / inverse: U U^-1 = 1 and det against numpy; the LAPACK
+//     pivoting convention is restated and pinned end to end by the det
+//     action of hmc.U*.ref (kappa_u1 = 1 there).
 //   * RNG streams (myrand, gaussian_rand_no): vs oracle/_ref stream dumps.
 //
 // Compile once per gauge group: gcc -O2 -DNCOL=N -shared -fPIC.

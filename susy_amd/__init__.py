@@ -45,6 +45,7 @@ EXPORTS = [
     "susy_fermion_action_dev", "susy_plaquette_dev", "susy_update", "susy_time_fermion_op",
     "susy_set_cg_options", "susy_ranmom", "susy_grsource_dev", "susy_validate_tables",
     "susy_prof_enable", "susy_prof_read", "susy_reseed",
+    "susy_set_verbose", "susy_stream_copy_dev",
 ]
 
 _libs = {}
@@ -82,7 +83,8 @@ def load(ncol):
         "susy_actions_dev": (i, [vp, dp]), "susy_fermion_action_dev": (i, [vp, i, i, dp]),
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
-        "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]),
+        "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
+        "susy_stream_copy_dev": (i, [vp, i, i]),
         "susy_prof_read": (i, [vp, i, dp, P(ctypes.c_long), P(ctypes.c_long)]),
         "susy_set_cg_options": (i, [i, i]), "susy_ranmom": (i, [vp]),
         "susy_grsource_dev": (i, [vp, i, i, i, ip]),
@@ -200,6 +202,9 @@ class Lattice:
 
     def fermion_op(self, hs, hd, sign, form=FORM_FP64):
         self._check(self.L.susy_fermion_op_dev(self.h, hs, hd, sign, form))
+
+    def stream_copy(self, hs, hd):
+        self._check(self.L.susy_stream_copy_dev(self.h, hs, hd))
 
     def dsq(self, hs, hd, form=FORM_FP64):
         self._check(self.L.susy_dsq_dev(self.h, hs, hd, form))

@@ -101,6 +101,18 @@ int tf_axpby(susy_ctx *c, double2 *y, double a, const double2 *x, double b) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+// Plain streaming copy of one Twist_Fermion field (calibration of the
+// memory-traffic counters: it reads and writes exactly 16 NCOL^2 V complex).
+__global__ void k_stream_copy(double2 *__restrict__ y, const double2 *__restrict__ x, size_t n) {
+  size_t i = (size_t)blockIdx.x * BB + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+int tf_stream_copy(susy_ctx *c, double2 *dst, const double2 *src) {
+  size_t n = tf_elems(c);
+  hipLaunchKernelGGL(k_stream_copy, dim3((n + BB - 1) / BB), dim3(BB), 0, c->s, dst, src, n);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 int tf_copy(susy_ctx *c, double2 *dst, const double2 *src) {
   HIPCHK(hipMemcpyAsync(dst, src, sizeof(double2) * tf_elems(c), hipMemcpyDeviceToDevice, c->s));
   return 0;

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <thread>
 
 #include "../../include/susy_rhmc15.h"
@@ -404,6 +405,14 @@ static int dsq64(susy_ctx *c, const double2 *s, double2 *d) {
   int nb;
   return dslash64(c, c->d_tmp, d, -1, s, f2, c->d_part, &nb);
 }
+extern "C" int susy_stream_copy_dev(susy_ctx *c, int hs, int hd) {
+  double2 *s = tfh(c, hs), *d = tfh(c, hd);
+  if (!s || !d) return SUSY_E_ARG;
+  RET(tf_stream_copy(c, d, s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}
+
 extern "C" int susy_dsq_dev(susy_ctx *c, int hs, int hd, int form) {
   double2 *s = tfh(c, hs), *d = tfh(c, hd);
   if (!s || !d) return SUSY_E_ARG;
@@ -666,13 +675,27 @@ static int traj_force(Traj &T, int root, double eps) {
   return 0;
 }
 
+static int g_verbose = 0;
+extern "C" int susy_set_verbose(int v) {
+  g_verbose = v;
+  return 0;
+}
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
 static int traj_cg(Traj &T, int root, int *iters) {
   susy_ctx *c = T.c;
-  double sr;
+  double sr, t0 = g_verbose ? now_s() : 0;
   int it = cg_multi(c, T.gsrc[root], T.psim[root].data(), c->p.Norder, c->shift4, c->p.niter,
                     c->p.error_per_site * c->p.error_per_site, &sr, T.form);
   if (it < 0) return it;
   *iters += it;
+  if (g_verbose) {
+    fprintf(stderr, "susy: cg root %d: %d iterations, %.3f s\n", root, it, now_s() - t0);
+    fflush(stderr);
+  }
   return 0;
 }
 
@@ -750,7 +773,16 @@ extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
   }
   int iters = 0;
   RET(ranmom(c));
-  for (int n = 0; n < Nr; n++) RET(grsource(c, T.gsrc[n], T.psim[n].data(), form, &iters));
+  for (int n = 0; n < Nr; n++) {
+    double t0 = g_verbose ? now_s() : 0;
+    int before = iters;
+    RET(grsource(c, T.gsrc[n], T.psim[n].data(), form, &iters));
+    if (g_verbose) {
+      fprintf(stderr, "susy: heatbath root %d: %d iterations, %.3f s\n", n, iters - before,
+              now_s() - t0);
+      fflush(stderr);
+    }
+  }
   for (int n = 0; n < Nr; n++) RET(traj_cg(T, n, &iters));
   double p0[6], p1[6];
   RET(traj_action(T, p0));

@@ -205,6 +205,7 @@ int read_scalars(susy_ctx *c, double *out, int n);
 int tf_dot_re(susy_ctx *c, const double2 *a, const double2 *b, double *out);
 int tf_axpby(susy_ctx *c, double2 *y, double a, const double2 *x, double b);
 int tf_copy(susy_ctx *c, double2 *dst, const double2 *src);
+int tf_stream_copy(susy_ctx *c, double2 *dst, const double2 *src);
 int tf_zero(susy_ctx *c, double2 *dst);
 int aos_to_soa(susy_ctx *c, const double *aos, double2 *soa, int nmat);
 int soa_to_aos(susy_ctx *c, const double2 *soa, double *aos, int nmat);

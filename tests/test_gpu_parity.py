@@ -163,3 +163,40 @@ def test_update_u_and_actions(ncol):
     a, ao = lat.actions(), O.actions()
     for k in range(4):
         assert abs(a[k] / ao[k] - 1) < 1e-12, k
+
+
+@pytest.mark.parametrize("ncol", [2, 4])
+def test_larger_volume(ncol):
+    """8^3 x 16: many site chunks per XCD and larger buffer offsets"""
+    say("large volume", ncol)
+    O, lat = pair(ncol, cfg=False, dims=(8, 8, 8, 16), PBC=-1, G=0.05, bmass=0.4, lam=2.0,
+                  kappa_u1=0.3)
+    lk = random_links(O.link_shape(), 41, 0.2)
+    O.set_links(lk)
+    lat.set_links(lk)
+    src = random_tf(O.tf_shape(), 42)
+    h = lat.tf_alloc(5)
+    lat.tf_upload(h, src)
+    for sign in (1, -1):
+        want = O.fermion_op(src, sign)
+        lat.fermion_op(h, h + 1, sign)
+        assert rel_err(lat.tf_download(h + 1), want) < 1e-12
+        lat.fermion_op(h, h + 1, sign, form=susy_amd.FORM_FP32)
+        assert rel_err(lat.tf_download(h + 1), want) < 1e-5
+    mom = random_tf(O.link_shape(), 43)
+    O.set_mom(mom)
+    lat.set_mom(mom)
+    assert abs(lat.gauge_force(0.02) / O.gauge_force(0.02) - 1) < 1e-12
+    assert rel_err(lat.get_mom() - mom, O.get_mom() - mom) < 1e-12
+    sol = random_tf((3,) + O.tf_shape(), 44)
+    amps = O.rhmc()[1:4]
+    O.set_mom(mom)
+    lat.set_mom(mom)
+    for j in range(3):
+        lat.tf_upload(h + 2 + j, sol[j])
+    n = lat.fermion_force(0.03, h + 2, amps)
+    assert abs(n / O.fermion_force(0.03, sol, amps) - 1) < 1e-10
+    assert rel_err(lat.get_mom() - mom, O.get_mom() - mom) < 1e-10
+    a, ao = lat.actions(), O.actions()
+    for k in range(4):
+        assert abs(a[k] / ao[k] - 1) < 1e-11, k
