@@ -198,6 +198,10 @@ int susy_time_fermion_op(susy_ctx *c, int h_src, int h_dst, int form,
 // susy_prof_read returns the average sampled duration in ms and the number
 // of launches for prec 0 (FP64 kernel) or 1 (FP32 kernel).
 int susy_prof_enable(susy_ctx *c, int every);
+/* Block order of the operator kernel (a tuning knob; results do not depend
+   on it): 0 = each XCD walks a contiguous range of sites, 1 (default) = each
+   XCD walks the same range of the even and the odd half of the site order. */
+int susy_set_site_order(susy_ctx *c, int paired);
 int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
                    long *sampled);
 

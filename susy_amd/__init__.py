@@ -45,7 +45,7 @@ EXPORTS = [
     "susy_fermion_action_dev", "susy_plaquette_dev", "susy_update", "susy_time_fermion_op",
     "susy_set_cg_options", "susy_ranmom", "susy_grsource_dev", "susy_validate_tables",
     "susy_prof_enable", "susy_prof_read", "susy_reseed",
-    "susy_set_verbose", "susy_stream_copy_dev",
+    "susy_set_verbose", "susy_stream_copy_dev", "susy_set_site_order",
 ]
 
 _libs = {}
@@ -84,6 +84,7 @@ def load(ncol):
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
         "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
+        "susy_set_site_order": (i, [vp, i]),
         "susy_stream_copy_dev": (i, [vp, i, i]),
         "susy_prof_read": (i, [vp, i, dp, P(ctypes.c_long), P(ctypes.c_long)]),
         "susy_set_cg_options": (i, [i, i]), "susy_ranmom": (i, [vp]),
@@ -259,6 +260,10 @@ class Lattice:
 
     def prof_enable(self, every):
         self._check(self.L.susy_prof_enable(self.h, every))
+
+    def set_site_order(self, paired):
+        """block order of the operator kernel (tuning only)"""
+        self._check(self.L.susy_set_site_order(self.h, int(paired)))
 
     def prof_read(self, prec):
         """(average ms of sampled operator-kernel launches, launches, sampled)"""

@@ -43,6 +43,37 @@ static int validate_tables(const PlaqTerms &pt, const ForceTerms &ft) {
         set_error("force term table entry (%d,%d) invalid", c, n);
         return SUSY_E_STATE;
       }
+  // The operator kernel applies the DbplusPtoP terms at double weight in
+  // place of DbplusPtoP + DbminusPtoP: every DbminusPtoP term must equal the
+  // DbplusPtoP term with the same output plaquette and c.
+  for (int p = 0; p < NPLAQ; p++)
+    for (int n = 0; n < 3; n++) {
+      int hit = 0;
+      for (int m = 0; m < 3; m++)
+        hit += pt.pp_c[p][m] == pt.pm_c[p][n] && pt.pp_de[p][m] == pt.pm_ab[p][n] &&
+               pt.pp_g1[p][m] == pt.pm_g1[p][n] && pt.pp_g2[p][m] == pt.pm_g2[p][n] &&
+               pt.pp_dt3[p][m] == pt.pm_dt2[p][n] && pt.pp_dt2[p][m] == pt.pm_dt3[p][n] &&
+               pt.pp_tr[p][m] == pt.pm_tr[p][n];
+      if (hit != 1) {
+        set_error("DbminusPtoP term (%d,%d) has no matching DbplusPtoP term", p, n);
+        return SUSY_E_STATE;
+      }
+    }
+  // Likewise the force kernel applies F1Q at double weight in place of
+  // F1Q + F2Q: the F2Q term of (a,b,d,e) must equal the F1Q term of (d,e,a,b).
+  for (int c = 0; c < NUMLINK; c++)
+    for (int n = 0; n < 6; n++) {
+      int hit = 0;
+      for (int m = 0; m < 6; m++)
+        hit += ft.ab[c][m] == ft.de[c][n] && ft.de[c][m] == ft.ab[c][n] &&
+               ft.g_abc[c][m] == ft.g_mab[c][n] && ft.g_mab[c][m] == ft.g_abc[c][n] &&
+               ft.dt3[c][m] == ft.dt2m[c][n] && ft.dt2m[c][m] == ft.dt3[c][n] &&
+               ft.perm1[c][m] == ft.perm2[c][n];
+      if (hit != 1) {
+        set_error("F2Q term (%d,%d) has no matching F1Q term", c, n);
+        return SUSY_E_STATE;
+      }
+    }
   return 0;
 }
 
@@ -141,8 +172,7 @@ static int ctx_setup(susy_ctx *c) {
   RET(dalloc(&c->f_aux, (size_t)6 * V));
   RET(dalloc(&c->d_tmp, tfn));
   RET(dalloc(&c->d_boundary, tfn));
-  int chunks = (V + DS_BLOCK - 1) / DS_BLOCK;
-  int ds_blocks = NTF * ((chunks + 7) / 8 * 8);
+  int ds_blocks = std::max(ds_grid(V, 0, nullptr), ds_grid(V, 1, nullptr));
   c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
   RET(dalloc(&c->d_part, (size_t)c->npart * 2));
   RET(dalloc(&c->d_scal, (size_t)64));
@@ -676,6 +706,15 @@ static int traj_force(Traj &T, int root, double eps) {
 }
 
 static int g_verbose = 0;
+extern "C" int susy_set_site_order(susy_ctx *c, int paired) {
+  if (!c || (paired != 0 && paired != 1)) {
+    set_error("susy_set_site_order: paired must be 0 or 1");
+    return SUSY_E_ARG;
+  }
+  c->site_order = paired;
+  return 0;
+}
+
 extern "C" int susy_set_verbose(int v) {
   g_verbose = v;
   return 0;

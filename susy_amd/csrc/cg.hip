@@ -6,18 +6,20 @@
 // The vector work is restructured for HBM bandwidth:
 //   * D^dag D p0, + sigma_0 p0 and the <p0, A p0> partial sums are one
 //     operator pair with a fused epilogue;
-//   * r += beta_0 A p0 and |r|^2 are one pass, which also stores r into a
-//     window of the last m residuals;
-//   * psi_0 -= beta_0 p0 and p0 = r + alpha_0 p0 are one pass;
-//   * the shifted systems j >= 1 are NOT touched every iteration.  Within a
-//     window, p_j and psi_j stay linear combinations of their window-start
-//     values and the stored residuals:
+//   * r += beta_0 A p0 and |r|^2 are one pass that writes the new r into
+//     the next slot of a ring of the last m residuals (r has no buffer of
+//     its own);
+//   * p0 = r + alpha_0 p0 is one pass;
+//   * no solution psi_j (the base system included) and no shifted direction
+//     p_j (j >= 1) is touched every iteration.  Within a window they stay
+//     linear combinations of their window-start values and the residuals
+//     of the ring:
 //       p_j   = A_j p_j(k0)        + sum_l C_jl r_l
 //       psi_j = psi_j(k0) + E_j p_j(k0) + sum_l D_jl r_l
-//     with scalar coefficients advanced on the host by the same recurrences;
-//     one fused flush applies them to all active shifts every m iterations.
-//   This replaces ~4 A field passes per iteration by (m + 4 A) / m.
-//
+//     with scalar coefficients advanced on the host by the same recurrences
+//     (zeta_0 = 1 for the base system, whose p0(k0) is kept by writing p0
+//     into the second of two buffers during the window); one fused flush
+//     applies them to all active systems every m iterations.
 // Precision forms:
 //   FP64: all vectors FP64 (the parity form).
 //   FP32: p0, r, A p0, the residual window and p_j in FP32 with the FP32
@@ -83,53 +85,56 @@ k_cg_init(const double2 *__restrict__ src, C *__restrict__ r, C *__restrict__ p0
   cg_block_sum(s, partial);
 }
 
-// r += beta0 * ap; win = r; partial |r|^2
+// rout = rin + beta0 * ap; partial |rout|^2.  rout is the next slot of the
+// residual window and rin the previous one (the same slot when the window
+// holds one residual: the update is elementwise, so in place is safe).
 template <typename C>
 __global__ void __launch_bounds__(CB)
-k_cg_r(C *__restrict__ r, const C *__restrict__ ap, C *__restrict__ win, double beta0,
-       size_t n, double *__restrict__ partial) {
+k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, double beta0, size_t n,
+       double *__restrict__ partial) {
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
-    C a = ap[i], v = r[i];
+    C a = ap[i], v = rin[i];
     v.x = fma(beta0, a.x, v.x);
     v.y = fma(beta0, a.y, v.y);
-    r[i] = v;
-    if (win) win[i] = v;
+    rout[i] = v;
     s += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
   }
   cg_block_sum(s, partial);
 }
 
-// psi0 -= beta0 p0; p0 = r + alpha0 p0
+// pout = r + alpha0 pin (pout is pin, or the other p0 buffer at the first
+// iteration of a window, which leaves the window-start p0 intact)
 template <typename C>
 __global__ void __launch_bounds__(CB)
-k_cg_p0(double2 *__restrict__ psi0, C *__restrict__ p0, const C *__restrict__ r,
-        double beta0, double alpha0, size_t n) {
+k_cg_p0(const C *pin, const C *__restrict__ r, C *pout, double alpha0, size_t n) {
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
-  C p = p0[i], rv = r[i];
-  double2 x = psi0[i];
-  x.x -= beta0 * (double)p.x;
-  x.y -= beta0 * (double)p.y;
-  psi0[i] = x;
+  C p = pin[i], rv = r[i];
   p.x = rv.x + alpha0 * p.x;
   p.y = rv.y + alpha0 * p.y;
-  p0[i] = p;
+  pout[i] = p;
 }
 
-// Window flush for the active shifts.  coef layout per shift slot k:
-// [A, E, C_0..C_{m-1}, D_0..D_{m-1}]
+// Window flush.  Slot k updates psi_j (j = sh[k]) and, for j >= 1, p_j:
+//   p_j   <- A p_j(k0) + sum_l C_l r_l
+//   psi_j <- psi_j + E p_j(k0) + sum_l D_l r_l
+// coef layout per slot: [A, E, C_0..C_{MAXWIN-1}, D_0..D_{MAXWIN-1}].  For the
+// base system (j = 0) p(k0) is the frozen window-start p0 and is not written.
 struct FlushArgs {
-  int nact, cnt;
-  int sh[MAXSH];        // shift index j (1..nsh-1) of slot k
+  int nact;
+  int sh[MAXSH];
+};
+struct PPtrs {
+  void *p[MAXSH];
 };
 template <typename C, int M>
 __global__ void __launch_bounds__(CB)
-k_cg_flush(C *__restrict__ pj, PsiPtrs psi, const C *__restrict__ win,
+k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win,
            const double *__restrict__ coef, FlushArgs fa, size_t n) {
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
-  // all M slots are read; slots beyond fa.cnt carry zero coefficients
+  // all M slots are read; slots beyond this window's count carry zero weights
   double2 w[M];
 #pragma unroll
   for (int l = 0; l < M; l++) w[l] = to64(win[(size_t)l * n + i]);
@@ -137,8 +142,8 @@ k_cg_flush(C *__restrict__ pj, PsiPtrs psi, const C *__restrict__ win,
   for (int k = 0; k < fa.nact; k++) {
     const int j = fa.sh[k];
     const double *cf = coef + k * stride;
-    C *pp = pj + (size_t)(j - 1) * n + i;
-    double2 p = to64(*pp);
+    C *pk = (C *)pp.p[k] + i;
+    double2 p = to64(*pk);
     double2 x = psi.p[j][i];
     double A = cf[0], E = cf[1];
     double2 np = make_double2(A * p.x, A * p.y);
@@ -152,7 +157,7 @@ k_cg_flush(C *__restrict__ pj, PsiPtrs psi, const C *__restrict__ win,
       x.x += dl * w[l].x;
       x.y += dl * w[l].y;
     }
-    *pp = from64<C>(np);
+    if (j > 0) *pk = from64<C>(np);
     psi.p[j][i] = x;
   }
 }
@@ -199,7 +204,8 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   const bool fp32 = sizeof(R) == sizeof(float);
   const size_t n = tf_elems(c);
   const int m = g_window;
-  // workspace: r, p0, ap, tmp, p_j (nsh-1), window (m) in the CG precision;
+  // workspace in the CG precision: two p0 buffers, ap, tmp, p_j (nsh-1) and
+  // the window of the last m residuals, which also holds the current r;
   // the FP32 form also uses two FP64 fields for the true residual
   size_t need = (size_t)(4 + (nsh - 1) + m) * n * sizeof(C);
   if (c->cg_ws_bytes < need) {
@@ -212,9 +218,15 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
     c->cg_ws_bytes = need;
   }
   C *base = (C *)c->cg_ws;
-  C *r = base, *p0 = base + n, *ap = base + 2 * n, *tmp = base + 3 * n;
+  C *p0buf[2] = {base, base + n};
+  C *ap = base + 2 * n, *tmp = base + 3 * n;
   C *pj = base + 4 * n;
   C *win = pj + (size_t)(nsh - 1) * n;
+  auto slot = [&](int l) { return win + (size_t)l * n; };
+  // r is the newest window slot; before the first iteration the last one
+  C *r = slot(m - 1);
+  // p0: pcur is the current direction, pfrozen its value at window start
+  C *pcur = p0buf[0], *pfrozen = p0buf[0];
   double2 *ap64 = nullptr, *tmp64 = nullptr;
   if (fp32) {
     if (!c->cg_t64) HIPCHK(hipMalloc(&c->cg_t64, 2 * n * sizeof(double2)));
@@ -227,7 +239,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   double sigma0 = shifts[0] + ((c->p.fmass > IMAG_TOL) ? c->p.fmass * c->p.fmass : 0.0);
 
   int nb = nblk(n);
-  hipLaunchKernelGGL((k_cg_init<C>), dim3(nb), dim3(CB), 0, c->s, src, r, p0, pj, nsh, psi,
+  hipLaunchKernelGGL((k_cg_init<C>), dim3(nb), dim3(CB), 0, c->s, src, r, pcur, pj, nsh, psi,
                      n, c->d_part);
   HIPCHK(hipGetLastError());
   int e = reduce_partials(c, c->d_part, nb, 0);
@@ -241,7 +253,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   for (int j = 0; j < nsh; j++) {
     zim1[j] = 1; zi[j] = 1; al[j] = 0; bim1[j] = 1; conv[j] = 0; zip1[j] = 1; bi[j] = 0;
   }
-  // window coefficients, per shift j: A, E, C[m], D[m]
+  // window coefficients, per system j (0 = base): A, E, C[m], D[m]
   const int stride = 2 + 2 * MAXWIN;
   if (!c->h_coef) {
     HIPCHK(hipHostMalloc((void **)&c->h_coef, sizeof(double) * stride * MAXSH));
@@ -257,16 +269,18 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
       for (int l = 0; l < MAXWIN; l++) Cw[j][l] = Dw[j][l] = 0;
     }
     cnt = 0;
+    pfrozen = pcur;
   };
   auto flush = [&]() -> int {
     FlushArgs fa;
+    PPtrs pp;
     fa.nact = 0;
-    fa.cnt = cnt;
-    for (int j = 1; j < nsh; j++) {
+    for (int j = 0; j < nsh; j++) {
       if (!touched[j]) continue;
       double *cf = h_coef + fa.nact * stride;
       cf[0] = A[j]; cf[1] = E[j];
       for (int l = 0; l < MAXWIN; l++) { cf[2 + l] = Cw[j][l]; cf[2 + MAXWIN + l] = Dw[j][l]; }
+      pp.p[fa.nact] = j == 0 ? (void *)pfrozen : (void *)(pj + (size_t)(j - 1) * n);
       fa.sh[fa.nact++] = j;
     }
     if (fa.nact > 0) {
@@ -274,11 +288,11 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
                             hipMemcpyHostToDevice, c->s));
       dim3 g((n + CB - 1) / CB), b(CB);
       switch (m) {
-        case 1: hipLaunchKernelGGL((k_cg_flush<C, 1>), g, b, 0, c->s, pj, psi, win, d_coef, fa, n); break;
-        case 4: hipLaunchKernelGGL((k_cg_flush<C, 4>), g, b, 0, c->s, pj, psi, win, d_coef, fa, n); break;
-        case 8: hipLaunchKernelGGL((k_cg_flush<C, 8>), g, b, 0, c->s, pj, psi, win, d_coef, fa, n); break;
-        case 16: hipLaunchKernelGGL((k_cg_flush<C, 16>), g, b, 0, c->s, pj, psi, win, d_coef, fa, n); break;
-        default: hipLaunchKernelGGL((k_cg_flush<C, 32>), g, b, 0, c->s, pj, psi, win, d_coef, fa, n); break;
+        case 1: hipLaunchKernelGGL((k_cg_flush<C, 1>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
+        case 4: hipLaunchKernelGGL((k_cg_flush<C, 4>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
+        case 8: hipLaunchKernelGGL((k_cg_flush<C, 8>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
+        case 16: hipLaunchKernelGGL((k_cg_flush<C, 16>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
+        default: hipLaunchKernelGGL((k_cg_flush<C, 32>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
       }
       HIPCHK(hipGetLastError());
       // the pinned coefficient buffer is reused by the next flush
@@ -320,7 +334,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   for (;;) {
     while (iteration < MaxCG && rsq > rsqstop) {
       int nbd;
-      if ((e = dsq(p0, ap, c->d_part, &nbd))) return e;
+      if ((e = dsq(pcur, ap, c->d_part, &nbd))) return e;
       if ((e = reduce_partials(c, c->d_part, nbd, 0))) return e;
       double cd;
       if ((e = read_scalars(c, &cd, 1))) return e;
@@ -336,20 +350,25 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
         zip1[j] /= c1 + c2;
         bi[j] = bi[0] * zip1[j] / zi[j];
       }
-      hipLaunchKernelGGL((k_cg_r<C>), dim3(nb), dim3(CB), 0, c->s, r, ap,
-                         win + (size_t)cnt * n, bi[0], n, c->d_part);
+      C *rnew = slot(cnt);
+      hipLaunchKernelGGL((k_cg_r<C>), dim3(nb), dim3(CB), 0, c->s, r, ap, rnew, bi[0], n,
+                         c->d_part);
       HIPCHK(hipGetLastError());
+      r = rnew;
       if ((e = reduce_partials(c, c->d_part, nb, 0))) return e;
       double rsqnew;
       if ((e = read_scalars(c, &rsqnew, 1))) return e;
       al[0] = rsqnew / rsq;
       for (int j = 1; j < nsh; j++)
         if (!conv[j]) al[j] = al[0] * zip1[j] * bi[j] / (zi[j] * bi[0]);
-      hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, psim[0],
-                         p0, r, bi[0], al[0], n);
+      C *pnext = pcur != pfrozen ? pcur : (pcur == p0buf[0] ? p0buf[1] : p0buf[0]);
+      hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, pcur, r,
+                         pnext, al[0], n);
       HIPCHK(hipGetLastError());
-      // advance the window coefficients of the shifted systems
-      for (int j = 1; j < nsh; j++) {
+      pcur = pnext;
+      // advance the window coefficients: psi_j -= beta_j p_j, then
+      // p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
+      for (int j = 0; j < nsh; j++) {
         if (conv[j]) continue;
         touched[j] = 1;
         E[j] -= bi[j] * A[j];

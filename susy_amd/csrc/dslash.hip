@@ -5,7 +5,7 @@
 // of one site and sums all terms landing on it:
 //   site  eta      : DbminusLtoS + detLtoS                 (10 matmuls)
 //   link  psi_b    : Dminus + DbplusStoL + detStoL          (10 matmuls)
-//   plaq  chi_ab   : Dplus + DbplusPtoP + DbminusPtoP       (16 matmuls)
+//   plaq  chi_ab   : Dplus + DbplusPtoP + DbminusPtoP       (10 matmuls)
 // so no output is written twice and no atomics are needed.  D^dagger uses
 // D^dagger v = -(D v^dagger)^dagger (utilities.c:780-788, 856-864): the
 // adjoint is applied on the loads and the stores, in registers.
@@ -38,6 +38,22 @@ int dslash_init_tables(const PlaqTerms &pt) {
 #ifndef DS_WAVES
 #define DS_WAVES 2
 #endif
+// dynamic LDS bytes requested per block: caps the blocks resident per CU
+// (160 KB of LDS per CU), and with it the sites in flight per XCD, whose
+// neighbourhood must fit the XCD's L2.  0 leaves occupancy to registers.
+#ifndef DS_OCC_LDS64
+#define DS_OCC_LDS64 0
+#endif
+#ifndef DS_OCC_LDS32
+#define DS_OCC_LDS32 0
+#endif
+// unroll factor of the term loops (a wave can issue the next product's
+// loads before the current product's FMAs when registers allow)
+#ifndef DS_UNROLL
+#define DS_UNROLL 1
+#endif
+#define DS_STR(x) #x
+#define DS_UNROLL_LOOP(n) _Pragma(DS_STR(unroll n))
 
 template <typename C, typename R>
 __device__ __forceinline__ void cfma(C &acc, R ar, R ai, C b) {
@@ -161,16 +177,27 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
          const int *__restrict__ nbr, const int *__restrict__ tco, int V, int nt,
-         R pbc, int doG, R G, int nchunks, int nchunks_pad,
+         R pbc, int doG, R G, DsOrder ord,
          const typename CT<R>::t *__restrict__ p0,
          R shift, double *__restrict__ partial) {
   typedef typename CT<R>::t C;
   const int L = blockIdx.x;
   const int q = L >> 3;                       // position in this XCD's queue
   const int comp = q % NTF;
-  const int chunk = (L & 7) * (nchunks_pad >> 3) + q / NTF;
-  const int s = chunk * DS_BLOCK + threadIdx.x;
-  const bool live = chunk < nchunks && s < V;
+  const int qq = q / NTF;                     // chunk position in the queue
+  int s;
+  bool live;
+  if (ord.paired) {
+    // even chunk k, odd chunk k, even chunk k+1, ... of this XCD's slab
+    const int par = qq & 1, chunk = (L & 7) * ord.slab + (qq >> 1);
+    const int i = chunk * DS_BLOCK + threadIdx.x;
+    s = par ? ord.half + i : i;
+    live = i < (par ? V - ord.half : ord.half);
+  } else {
+    const int chunk = (L & 7) * ord.slab + qq;
+    s = chunk * DS_BLOCK + threadIdx.x;
+    live = s < V;
+  }
 
   const uint32_t cs = sizeof(C);
   const uint32_t pl = (uint32_t)V * cs;     // one element plane
@@ -192,7 +219,7 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
 
     if (comp == 0) {
       // ---- site output: 1/2 sum_a [psi_a(x) Ubar_a(x) - bc(-a) Ubar_a(x-a) psi_a(x-a)]
-#pragma unroll 1
+      DS_UNROLL_LOOP(DS_UNROLL)
       for (int a = 0; a < NUMLINK; a++) {
         const int xm = nb(g_bwd(a));
         mmacc<R, DAG, true>(acc, (R)0.5, S(1 + a, s), Ul(a, s), pl);
@@ -231,7 +258,7 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       // ---- link output b
       const int b = comp - 1;
       const int xb = nb(g_fwd(b));
-#pragma unroll 1
+      DS_UNROLL_LOOP(DS_UNROLL)
       for (int a = 0; a < NUMLINK; a++) {
         if (a == b) continue;
         // Dminus: U_a(x+b) chi_ab(x) - bc(-a) chi_ab(x-a) U_a(x-a)
@@ -286,28 +313,21 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       mmacc<R, DAG, false>(acc, (R)-1, S(1 + b, s), Ul(a, xb), pl);
       mmacc<R, false, DAG>(acc, -bcsign<R>(t, link_tau(b), nt, pbc), Ul(b, s), S(1 + a, xb), pl);
       mmacc<R, DAG, false>(acc, (R)1, S(1 + a, s), Ul(b, xa), pl);
-#pragma unroll 1
+      // DbplusPtoP + DbminusPtoP.  Each DbminusPtoP term landing on (a,b)
+      // coincides with the DbplusPtoP term of the same c: with (d,e) the
+      // other two links, x-d-e = x+a+b+c and x-d-e-c = x+a+b (the five e_a
+      // sum to zero), perm(d,e,c,a,b) = perm(a,b,c,d,e), and the temporal
+      // offsets agree for the same reason.  The DbplusPtoP terms are applied
+      // at double weight (checked term by term when the tables are built).
+      DS_UNROLL_LOOP(DS_UNROLL)
       for (int n = 0; n < 3; n++) {
-        // DbplusPtoP
         const int c = c_pt.pp_c[p][n];
         const int pc = 6 + c_pt.pp_de[p][n];
         const int x1 = nb(c_pt.pp_g1[p][n]), x2 = nb(c_pt.pp_g2[p][n]), xmc = nb(g_bwd(c));
-        const R tr = (R)c_pt.pp_tr[p][n];
+        const R tr = (R)(2.0f * c_pt.pp_tr[p][n]);
         mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pp_dt3[p][n], nt, pbc), S(pc, x2),
                             Ul(c, x1), pl);
         mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pp_dt2[p][n], nt, pbc), Ul(c, xmc),
-                            S(pc, x1), pl);
-      }
-#pragma unroll 1
-      for (int n = 0; n < 3; n++) {
-        // DbminusPtoP
-        const int c = c_pt.pm_c[p][n];
-        const int pc = 6 + c_pt.pm_ab[p][n];
-        const int x2 = nb(c_pt.pm_g2[p][n]), x1 = nb(c_pt.pm_g1[p][n]), xmc = nb(g_bwd(c));
-        const R tr = (R)c_pt.pm_tr[p][n];
-        mmacc<R, DAG, true>(acc, tr * bcsign<R>(t, c_pt.pm_dt2[p][n], nt, pbc), S(pc, x2),
-                            Ul(c, x1), pl);
-        mmacc<R, true, DAG>(acc, -tr * bcsign<R>(t, c_pt.pm_dt3[p][n], nt, pbc), Ul(c, xmc),
                             S(pc, x1), pl);
       }
     }
@@ -367,13 +387,12 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
       hipLaunchKernelGGL((k_dslash_prep<R, true>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V);
     HIPCHK(hipGetLastError());
   }
-  int nchunks = (V + DS_BLOCK - 1) / DS_BLOCK;
-  int padded = (nchunks + 7) / 8 * 8;
-  int nb = padded * NTF;
+  DsOrder ord;
+  const int nb = ds_grid(V, c->site_order, &ord);
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \
-                nchunks, padded, p0, shift, partial
+                ord, p0, shift, partial
   const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->prof_every > 0 && (c->prof_launch[pi]++ % c->prof_every) == 0) {
@@ -381,16 +400,17 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c->s));
   }
+  const unsigned lds = sizeof(R) == sizeof(double) ? DS_OCC_LDS64 : DS_OCC_LDS32;
   if (p0 == nullptr) {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_BLOCK), 0, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(DS_BLOCK), 0, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
   } else {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(DS_BLOCK), 0, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(DS_BLOCK), 0, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
   }
 #undef DS_ARGS
   HIPCHK(hipGetLastError());

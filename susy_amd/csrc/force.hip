@@ -135,22 +135,23 @@ k_force_assemble(const double2 *__restrict__ S, const double2 *__restrict__ P,
     fmm<true, false>(X, bm * sg, Sc + s, Pl + xm, V);
     fmm<false, true>(X, -sg, Pl + xn, Sc + xn, V);
   }
-  // ---- Q-closed F1Q / F2Q on link c = mu: f_U += -1/2 T^dag, X += -1/2 T
+  // ---- Q-closed F1Q / F2Q on link c = mu: f_U += -1/2 T^dag, X += -1/2 T.
+  // The F2Q term of (a,b,d,e) equals the F1Q term of (d,e,a,b): x-a-b =
+  // x+c+d+e, perm(a,b,c,d,e) = perm(d,e,c,a,b) and the two temporal offsets
+  // trade places, so the F1Q terms are applied at double weight (checked
+  // term by term when the tables are built).
   for (int n = 0; n < 6; n++) {
     const int ab = c_ft.ab[mu][n], de = c_ft.de[mu][n];
     const int xabc = nbr[c_ft.g_abc[mu][n] * V + s];
     const int xmab = nbr[c_ft.g_mab[mu][n] * V + s];
     const double b3 = bcs(t, c_ft.dt3[mu][n], nt, pbc) * bm;
     const double b2 = bcs(t, c_ft.dt2m[mu][n], nt, pbc) * bm;
-    const double2 *Sab = S + (6 + ab) * M, *Sde = S + (6 + de) * M;
-    const double2 *Pab = P + (6 + ab) * M, *Pde = P + (6 + de) * M;
-    const double p1 = c_ft.perm1[mu][n], p2 = c_ft.perm2[mu][n];
+    const double2 *Sab = S + (6 + ab) * M;
+    const double2 *Pde = P + (6 + de) * M;
+    const double p1 = c_ft.perm1[mu][n];
     // F1Q: T = p1 b3 Pde^dag(x+a+b+c) Sab(x+c) - p1 b2 Sab(x-a-b) Pde^dag(x+c)
-    fmm<true, false>(X, -0.5 * p1 * b3, Pde + xabc, Sab + xm, V);
-    fmm<false, true>(X, 0.5 * p1 * b2, Sab + xmab, Pde + xm, V);
-    // F2Q: T = p2 b2 Pab^dag(x-a-b) Sde(x+c) - p2 b3 Sde(x+a+b+c) Pab^dag(x+c)
-    fmm<true, false>(X, -0.5 * p2 * b2, Pab + xmab, Sde + xm, V);
-    fmm<false, true>(X, 0.5 * p2 * b3, Sde + xabc, Pab + xm, V);
+    fmm<true, false>(X, -p1 * b3, Pde + xabc, Sab + xm, V);
+    fmm<false, true>(X, p1 * b2, Sab + xmab, Pde + xm, V);
   }
   if (doG) {
     // determinant terms: X^dag += z U^-1 - U^-1 W U^-1, i.e.

@@ -31,6 +31,33 @@
 #endif
 #define C2 1.0
 
+// Block order of the fused operator kernel.  Hardware block ids are dealt to
+// the 8 XCDs round-robin; each XCD walks one slab of `slab` site chunks.
+//   paired = 0: a slab is a contiguous range of site indices;
+//   paired = 1: a slab is the same chunk range in the even half [0, half) and
+//     the odd half [half, V) of the even-first site order, even and odd chunks
+//     alternating, so a site's nearest neighbours (opposite parity, same t
+//     range) are fetched into the same L2.
+struct DsOrder {
+  int paired, slab, half;
+};
+// grid size (blocks) of the operator kernel, and its order parameters
+inline int ds_grid(int V, int paired, DsOrder *o) {
+  DsOrder d;
+  d.paired = paired;
+  d.half = V / 2;
+  if (paired) {
+    const int chunks = (V - d.half + DS_BLOCK - 1) / DS_BLOCK;   // odd half is the larger
+    d.slab = (chunks + 7) / 8;
+    if (o) *o = d;
+    return 8 * d.slab * 2 * NTF;
+  }
+  const int chunks = (V + DS_BLOCK - 1) / DS_BLOCK;
+  d.slab = (chunks + 7) / 8;
+  if (o) *o = d;
+  return 8 * d.slab * NTF;
+}
+
 // ---------------------------------------------------------------------
 // Complex helpers for double2 / float2
 template <typename R> struct CT;
@@ -55,6 +82,9 @@ struct PlaqTerms {
   float pp_tr[NPLAQ][3];
   // DbminusPtoP terms whose output (d,e) is plaquette p: dest_de +=
   //   tr [bc2' chi_ab(x-a-b) Ubar_c(x-a-b-c) - bc3' Ubar_c(x-c) chi_ab(x-a-b-c)]
+  // Each coincides with the DbplusPtoP term of the same p and c; the kernel
+  // applies DbplusPtoP at double weight and these entries serve only the
+  // set-up check of that coincidence.
   int pm_c[NPLAQ][3], pm_ab[NPLAQ][3], pm_g2[NPLAQ][3], pm_g1[NPLAQ][3];
   int pm_dt2[NPLAQ][3], pm_dt3[NPLAQ][3];
   float pm_tr[NPLAQ][3];
@@ -162,6 +192,7 @@ struct susy_ctx {
   int total_iters = 0;
   // sampled HIP-event timing of the operator kernel launches (per precision:
   // index 0 FP64, 1 FP32): every prof_every-th launch is bracketed by events
+  int site_order = 1;          // DsOrder::paired of the operator kernel
   int prof_every = 0;
   long prof_launch[2] = {0, 0};
   std::vector<hipEvent_t> prof_ev[2];
