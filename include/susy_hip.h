@@ -111,6 +111,7 @@ typedef struct {
 enum {
   SUSY_FORM_FP64 = 0,            // plain FP64 everywhere (parity form)
   SUSY_FORM_FP32 = 1,            // FP32 DSq + CG with FP64 reliable updates
+  SUSY_FORM_SURROGATE = 4,       // flag: learned surrogate force on the inner MD level
 };
 
 int susy_ncol(void);
@@ -202,6 +203,34 @@ int susy_prof_enable(susy_ctx *c, int every);
    on it): 0 = each XCD walks a contiguous range of sites, 1 (default) = each
    XCD walks the same range of the even and the odd half of the site order. */
 int susy_set_site_order(susy_ctx *c, int paired);
+
+/* Learned surrogate action S_theta(U) (no counterpart in the reference; its
+   judge is the plain fermion_force).  Two gauge-invariant convolution layers
+   (16 channels, bfloat16 weights, FP32 accumulation) over plaquette and
+   1x2-loop traces; F_theta = dS_theta/dUbar in the convention of
+   fermion_force.  With SUSY_FORM_SURROGATE, susy_update adds F_theta to every
+   inner (gauge) force and removes it from every outer (fermion) kick, so the
+   outer level integrates S_F - S_theta and the exact action still decides
+   the Metropolis test. */
+int susy_sg_nparams(void);
+int susy_sg_init(susy_ctx *c, int seed, double scale);
+int susy_sg_set_params(susy_ctx *c, const float *theta);
+int susy_sg_get_params(susy_ctx *c, float *theta);
+/* S_theta and (if F_aos != NULL) F_theta at the context's links, AoS (V,5,N,N) */
+int susy_sg_eval(susy_ctx *c, double *action, double *F_aos);
+/* the 65 features as planes [feature][site] */
+int susy_sg_features(susy_ctx *c, float *phi);
+/* force-matching loss |F_theta - F|^2 / |F|^2 and its parameter gradient */
+int susy_sg_loss_grad(susy_ctx *c, const double *U_aos, const double *F_aos, double *loss,
+                      double *grad);
+/* record (links, fermion force) at every outer kick of susy_update, keeping
+   the last `capacity` pairs (0 stops); susy_sg_pairs = pairs held */
+int susy_sg_record(susy_ctx *c, int capacity);
+int susy_sg_pairs(susy_ctx *c);
+int susy_sg_pair_loss(susy_ctx *c, int i, double *loss);
+/* Adam over recorded pairs [first, first + count) */
+int susy_sg_train(susy_ctx *c, int first, int count, int steps, int batch, double lr, int seed,
+                  double *loss_hist);
 int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
                    long *sampled);
 

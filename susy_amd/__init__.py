@@ -16,6 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 
 FORM_FP64 = 0
 FORM_FP32 = 1
+FORM_SURROGATE = 4      # flag: learned surrogate force on the inner MD level
 NGATHER = 74
 
 
@@ -46,6 +47,9 @@ EXPORTS = [
     "susy_set_cg_options", "susy_ranmom", "susy_grsource_dev", "susy_validate_tables",
     "susy_prof_enable", "susy_prof_read", "susy_reseed",
     "susy_set_verbose", "susy_stream_copy_dev", "susy_set_site_order",
+    "susy_sg_nparams", "susy_sg_init", "susy_sg_set_params", "susy_sg_get_params",
+    "susy_sg_eval", "susy_sg_features", "susy_sg_loss_grad", "susy_sg_record", "susy_sg_pairs",
+    "susy_sg_pair_loss", "susy_sg_train",
 ]
 
 _libs = {}
@@ -96,6 +100,13 @@ def load(ncol):
         "fermion_op": (None, [vp, vp, i]), "DSq": (None, [vp, vp]),
         "congrad_multi": (i, [vp, P(vp), i, d, dp]),
         "fermion_force": (d, [d, vp, P(vp)]), "gauge_force": (d, [d]),
+        "susy_sg_nparams": (i, []), "susy_sg_init": (i, [vp, i, d]),
+        "susy_sg_set_params": (i, [vp, P(ctypes.c_float)]),
+        "susy_sg_get_params": (i, [vp, P(ctypes.c_float)]),
+        "susy_sg_eval": (i, [vp, dp, dp]), "susy_sg_features": (i, [vp, P(ctypes.c_float)]),
+        "susy_sg_loss_grad": (i, [vp, dp, dp, dp, dp]), "susy_sg_record": (i, [vp, i]),
+        "susy_sg_pairs": (i, [vp]), "susy_sg_pair_loss": (i, [vp, i, dp]),
+        "susy_sg_train": (i, [vp, i, i, i, i, d, i, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -272,6 +283,61 @@ class Lattice:
                                           ctypes.byref(k)))
         return ms.value, n.value, k.value
 
+    # ---- learned surrogate
+    def sg_init(self, seed=1, scale=1.0):
+        self._check(self.L.susy_sg_init(self.h, seed, scale))
+
+    def sg_set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        self._check(self.L.susy_sg_set_params(self.h, t.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+
+    def sg_get_params(self):
+        t = np.zeros(self.L.susy_sg_nparams(), dtype=np.float32)
+        self._check(self.L.susy_sg_get_params(self.h, t.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+        return t
+
+    def sg_eval(self, force=True):
+        """(S_theta, F_theta in AoS (V, 5, N, N) or None) at the current links"""
+        S = ctypes.c_double()
+        F = np.zeros(self.link_shape(), dtype=np.complex128) if force else None
+        self._check(self.L.susy_sg_eval(self.h, ctypes.byref(S),
+                                        _dp(F) if force else None))
+        return S.value, F
+
+    def sg_features(self):
+        """features as (V, 65)"""
+        phi = np.zeros((65, self.V), dtype=np.float32)
+        self._check(self.L.susy_sg_features(self.h, phi.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float))))
+        return phi.T.copy()
+
+    def sg_loss_grad(self, U, F):
+        u = np.ascontiguousarray(U, dtype=np.complex128)
+        f = np.ascontiguousarray(F, dtype=np.complex128)
+        loss = ctypes.c_double()
+        g = np.zeros(self.L.susy_sg_nparams())
+        self._check(self.L.susy_sg_loss_grad(self.h, _dp(u), _dp(f), ctypes.byref(loss), _dp(g)))
+        return loss.value, g
+
+    def sg_record(self, capacity):
+        self._check(self.L.susy_sg_record(self.h, capacity))
+
+    def sg_pairs(self):
+        return self._check(self.L.susy_sg_pairs(self.h))
+
+    def sg_pair_loss(self, i):
+        loss = ctypes.c_double()
+        self._check(self.L.susy_sg_pair_loss(self.h, i, ctypes.byref(loss)))
+        return loss.value
+
+    def sg_train(self, first, count, steps, batch=1, lr=1e-3, seed=1):
+        hist = np.zeros(max(steps, 1))
+        self._check(self.L.susy_sg_train(self.h, first, count, steps, batch, lr, seed,
+                                         _dp(hist)))
+        return hist[:steps]
+
     def reseed(self, seed):
         self._check(self.L.susy_reseed(self.h, seed))
 
@@ -284,7 +350,7 @@ class Lattice:
         return it.value
 
 
-def set_cg_options(ncol, window=16, reliable=50):
+def set_cg_options(ncol, window=32, reliable=50):
     r = load(ncol).susy_set_cg_options(window, reliable)
     if r < 0:
         raise RuntimeError(load(ncol).susy_last_error().decode())

@@ -1,6 +1,8 @@
 // Twist_Fermion BLAS-1 (reference: 4d_Q16/susy/library_util.c:53-168),
 // deterministic reductions, and the AoS <-> SoA transposes used at the
 // host boundary.
+#include <string.h>
+
 #include "susy_internal.h"
 
 #define BB 256
@@ -9,19 +11,20 @@ size_t tf_elems(const susy_ctx *c) { return (size_t)NTF * NN * c->V; }
 
 // One block sums n partials in a fixed order with Neumaier compensation and
 // leaves the result in d_scal[slot].
-__global__ void __launch_bounds__(BB)
+#define RB 1024
+__global__ void __launch_bounds__(RB)
 k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
   double s = 0, comp = 0;
-  for (int i = threadIdx.x; i < n; i += BB) {
+  for (int i = threadIdx.x; i < n; i += RB) {
     double v = part[i], t = s + v;
     comp += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
     s = t;
   }
-  __shared__ double rs[BB], rc[BB];
+  __shared__ double rs[RB], rc[RB];
   rs[threadIdx.x] = s;
   rc[threadIdx.x] = comp;
   __syncthreads();
-  for (int w = BB / 2; w > 0; w >>= 1) {
+  for (int w = RB / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       double a = rs[threadIdx.x], b = rs[threadIdx.x + w], t = a + b;
       double e = (fabs(a) >= fabs(b)) ? (a - t) + b : (b - t) + a;
@@ -34,14 +37,21 @@ k_reduce(const double *__restrict__ part, int n, double *__restrict__ out) {
 }
 
 int reduce_partials(susy_ctx *c, const double *part, int n, int slot) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BB), 0, c->s, part, n, c->d_scal + slot);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, c->s, part, n, c->d_scal + slot);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+// Reads back d_scal[0..n) through the pinned staging buffer (a pageable
+// destination would make the runtime stage the copy itself).
 int read_scalars(susy_ctx *c, double *out, int n) {
-  HIPCHK(hipMemcpyAsync(out, c->d_scal, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
+  if (n < 1 || n > 64) {
+    set_error("read_scalars: %d scalars requested (1..64)", n);
+    return SUSY_E_ARG;
+  }
+  HIPCHK(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * n, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
+  memcpy(out, c->h_scal, sizeof(double) * n);
   return 0;
 }
 

@@ -89,6 +89,7 @@ extern "C" int susy_validate_tables(void) {
 }
 
 static void free_ctx(susy_ctx *c) {
+  sg_free(c);
   for (int p = 0; p < 2; p++)
     for (hipEvent_t e : c->prof_ev[p]) (void)hipEventDestroy(e);
   void *ptrs[] = {c->d_nbr, c->d_t, c->d_links, c->d_mom, c->d_old_links, c->d_old_mom,
@@ -103,6 +104,7 @@ static void free_ctx(susy_ctx *c) {
   for (double2 *p : c->traj_pool)
     if (p) (void)hipFree(p);
   if (c->h_coef) (void)hipHostFree(c->h_coef);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
 }
@@ -176,6 +178,7 @@ static int ctx_setup(susy_ctx *c) {
   c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
   RET(dalloc(&c->d_part, (size_t)c->npart * 2));
   RET(dalloc(&c->d_scal, (size_t)64));
+  HIPCHK(hipHostMalloc((void **)&c->h_scal, sizeof(double) * 64));
   HIPCHK(hipMemcpy(c->d_nbr, c->geo.gathers.data(), sizeof(int) * NGATHER * V,
                    hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_t, c->geo.t.data(), sizeof(int) * V, hipMemcpyHostToDevice));
@@ -672,7 +675,8 @@ extern "C" int susy_grsource_dev(susy_ctx *c, int hsrc, int hpsim, int form, int
 // One trajectory (update_o.c:20-282)
 struct Traj {
   susy_ctx *c;
-  int form;
+  int form;             // precision of the CG (SUSY_FORM_FP64 / SUSY_FORM_FP32)
+  bool sg = false;      // learned surrogate on the inner level
   std::vector<double2 *> gsrc;
   std::vector<std::vector<double2 *>> psim;
   double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
@@ -684,13 +688,19 @@ static int update_gauge_step(Traj &T, double eps, double *norm) {
   susy_ctx *c = T.c;
   int n = c->p.nstep_gauge;
   double nm, acc;
-  RET(gauge_force_dev(c, eps * OM_LAMBDA, &acc));
+  // the surrogate mode adds the learned force to every inner force
+  auto inner = [&](double e, double *nrm) -> int {
+    RET(gauge_force_dev(c, e, nrm));
+    if (T.sg) RET(sg_force_dev(c, e));
+    return 0;
+  };
+  RET(inner(eps * OM_LAMBDA, &acc));
   for (int i = 1; i <= n; i++) {
     RET(update_u_dev(c, 0.5 * eps));
-    RET(gauge_force_dev(c, eps * OM_LAMBDA_MID, &nm));
+    RET(inner(eps * OM_LAMBDA_MID, &nm));
     acc += nm;
     RET(update_u_dev(c, 0.5 * eps));
-    RET(gauge_force_dev(c, eps * (i < n ? OM_TWO_LAMBDA : OM_LAMBDA), &nm));
+    RET(inner(eps * (i < n ? OM_TWO_LAMBDA : OM_LAMBDA), &nm));
     acc += nm;
   }
   *norm = acc / n;
@@ -738,27 +748,38 @@ static int traj_cg(Traj &T, int root, int *iters) {
   return 0;
 }
 
+// Outer (fermion) kick of size eps at the current links: the solves (unless
+// the caller's psim already belong to these links), every root's force, and
+// in the surrogate mode the removal of the learned force that the inner
+// level applies, so that the outer level integrates S_F - S_theta.
+static int outer_kick(Traj &T, double eps, bool solve, int *iters) {
+  susy_ctx *c = T.c;
+  RET(sg_record_begin(c));
+  for (int n = 0; n < c->p.Nroot; n++) {
+    if (solve) RET(traj_cg(T, n, iters));
+    RET(traj_force(T, n, eps));
+    RET(sg_record_root(c));
+  }
+  RET(sg_record_end(c));
+  if (T.sg) RET(sg_force_dev(c, -eps));
+  return 0;
+}
+
 static int update_step(Traj &T, int *iters) {
   susy_ctx *c = T.c;
   const double f_eps = c->p.traj_length / (double)c->p.nstep;
   const double g_eps = f_eps / (double)(2.0 * c->p.nstep_gauge);
   double tr;
-  for (int n = 0; n < c->p.Nroot; n++) RET(traj_force(T, n, f_eps * OM_LAMBDA));
+  RET(outer_kick(T, f_eps * OM_LAMBDA, false, iters));
   for (int s = 1; s <= c->p.nstep; s++) {
     RET(update_gauge_step(T, g_eps, &tr));
     T.gnorm += tr;
     if (tr > T.max_gf) T.max_gf = tr;
-    for (int n = 0; n < c->p.Nroot; n++) {
-      RET(traj_cg(T, n, iters));
-      RET(traj_force(T, n, f_eps * OM_LAMBDA_MID));
-    }
+    RET(outer_kick(T, f_eps * OM_LAMBDA_MID, true, iters));
     RET(update_gauge_step(T, g_eps, &tr));
     T.gnorm += tr;
     if (tr > T.max_gf) T.max_gf = tr;
-    for (int n = 0; n < c->p.Nroot; n++) {
-      RET(traj_cg(T, n, iters));
-      RET(traj_force(T, n, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA)));
-    }
+    RET(outer_kick(T, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA), true, iters));
   }
   return 0;
 }
@@ -794,7 +815,16 @@ extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
   if (!c) return SUSY_E_STATE;
   Traj T;
   T.c = c;
-  T.form = form;
+  T.form = form & SUSY_FORM_FP32;
+  T.sg = (form & SUSY_FORM_SURROGATE) != 0;
+  if ((form & ~(SUSY_FORM_FP32 | SUSY_FORM_SURROGATE)) != 0) {
+    set_error("susy_update: unknown form bits 0x%x", form);
+    return SUSY_E_ARG;
+  }
+  if (T.sg && !c->sg) {
+    set_error("susy_update: surrogate form without susy_sg_init");
+    return SUSY_E_STATE;
+  }
   const int Nr = c->p.Nroot, No = c->p.Norder;
   // per-trajectory fields are kept in the context across calls
   std::vector<double2 *> &pool = c->traj_pool;
@@ -815,7 +845,7 @@ extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
   for (int n = 0; n < Nr; n++) {
     double t0 = g_verbose ? now_s() : 0;
     int before = iters;
-    RET(grsource(c, T.gsrc[n], T.psim[n].data(), form, &iters));
+    RET(grsource(c, T.gsrc[n], T.psim[n].data(), T.form, &iters));
     if (g_verbose) {
       fprintf(stderr, "susy: heatbath root %d: %d iterations, %.3f s\n", n, iters - before,
               now_s() - t0);

@@ -183,7 +183,7 @@ static int nblk(size_t n) {
   return (int)(nb < 2048 ? nb : 2048);
 }
 
-static int g_window = 16;     // residuals kept per window (<= MAXWIN)
+static int g_window = 32;     // residuals kept per window (<= MAXWIN)
 static int g_reliable = 50;   // FP32 form: iterations between true residuals
 
 extern "C" int susy_set_cg_options(int window, int reliable) {

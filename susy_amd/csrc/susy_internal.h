@@ -140,6 +140,8 @@ double prn_next(Prn *p);
 
 // ---------------------------------------------------------------------
 // Context
+struct SgState;   // learned surrogate (surrogate.hip)
+
 struct susy_ctx {
   susy_params p;
   Geometry geo;
@@ -175,6 +177,7 @@ struct susy_ctx {
   double *d_part = nullptr;         // per-block partial sums
   int npart = 0;
   double *d_scal = nullptr;         // final reduced scalars
+  double *h_scal = nullptr;         // pinned host copy of d_scal (64 doubles)
   // Twist_Fermion vectors
   std::vector<double2 *> tf;
   // CG workspace (allocated on demand, grown as needed)
@@ -193,6 +196,7 @@ struct susy_ctx {
   // sampled HIP-event timing of the operator kernel launches (per precision:
   // index 0 FP64, 1 FP32): every prof_every-th launch is bracketed by events
   int site_order = 1;          // DsOrder::paired of the operator kernel
+  SgState *sg = nullptr;       // learned surrogate state (susy_sg_init)
   int prof_every = 0;
   long prof_launch[2] = {0, 0};
   std::vector<hipEvent_t> prof_ev[2];
@@ -247,4 +251,11 @@ int cg_multi(susy_ctx *c, const double2 *src, double2 *const *psim, int nshift,
              int form);
 
 size_t tf_elems(const susy_ctx *c);   // NTF * NN * V complex
+
+// learned surrogate (surrogate.hip)
+void sg_free(susy_ctx *c);
+int sg_force_dev(susy_ctx *c, double eps);   // mom -= eps F_theta at the links
+int sg_record_begin(susy_ctx *c);            // training pairs of one outer step
+int sg_record_root(susy_ctx *c);
+int sg_record_end(susy_ctx *c);
 #endif  // SUSY_INTERNAL_H

@@ -100,7 +100,7 @@ def test_congrad_multi(ncol, window):
     assert abs(sz / sz_o - 1) < 1e-6
     for j in range(15):
         assert rel_err(lat.tf_download(hs + 1 + j), ps_o[j]) < 1e-10, j
-    susy_amd.set_cg_options(ncol, window=16)
+    susy_amd.set_cg_options(ncol, window=32)
 
 
 @pytest.mark.parametrize("ncol", [2, 3, 4])
