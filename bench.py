@@ -69,6 +69,11 @@ def measured_traffic():
         return None
 
 
+def finite_or_none(x):
+    x = float(x)
+    return x if x == x and abs(x) < 1e300 else None
+
+
 def run_form(lat, form, steps, start_links, prof_every):
     """`steps` trajectories from start_links with freshly seeded generators"""
     lat.set_links(start_links)
@@ -82,7 +87,7 @@ def run_form(lat, form, steps, start_links, prof_every):
               f"{int(stats[-1][3])} iters {int(stats[-1][4])} "
               f"t {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    prec = 1 if form == 1 else 0
+    prec = form & 1
     avg_ms, launches, sampled = lat.prof_read(prec)
     lat.prof_enable(0)
     return stats, t1 - t0, (avg_ms, launches, sampled), lat.plaquette()
@@ -199,7 +204,9 @@ def main():
                 "accepted_traj_per_hour": 3600.0 * acc / secs,
                 "acceptance": acc / len(stats),
                 "dH": [float(s[2]) for s in stats],
-                "exp_minus_dH": float(np.mean([np.exp(-s[2]) for s in stats])),
+                # null when a cold-start transient (dH << 0) overflows exp(-dH)
+                "exp_minus_dH": finite_or_none(np.mean([np.exp(min(-s[2], 700.0))
+                                                        for s in stats])),
                 "cg_iters": [int(s[4]) for s in stats],
                 "s_per_traj": secs / len(stats),
                 "plaquette_end": list(plaq),
@@ -230,7 +237,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(),
-                     "kernel": "k_dslash<float> (fermion_op, FP32)",
+                     "kernel": "k_dslash<float, false, 0>: D = fermion_op(sign +1), FP32",
                      "launches": prof32[1], "sampled": prof32[2], "avg_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes32},
         "forms": {"fp32": f32, "fp64": f64,

@@ -194,10 +194,11 @@ int susy_update(susy_ctx *c, int form, double *stats);
 // library's own stream).  Returns average milliseconds per call.
 int susy_time_fermion_op(susy_ctx *c, int h_src, int h_dst, int form,
                          int reps, double *ms_per_call);
-// Sampled HIP-event timing of every operator-kernel launch the context makes
-// (in CG, forces, trajectories): every `every`-th launch is timed (0 = off).
+// Sampled HIP-event timing of the operator kernel D (fermion_op sign +1, the
+// first half of DSq) wherever the context launches it (CG, forces,
+// trajectories): every `every`-th such launch is timed (0 = off).
 // susy_prof_read returns the average sampled duration in ms and the number
-// of launches for prec 0 (FP64 kernel) or 1 (FP32 kernel).
+// of D launches for prec 0 (FP64 kernel) or 1 (FP32 kernel).
 int susy_prof_enable(susy_ctx *c, int every);
 /* Block order of the operator kernel (a tuning knob; results do not depend
    on it): 0 = each XCD walks a contiguous range of sites, 1 (default) = each

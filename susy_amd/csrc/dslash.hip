@@ -393,9 +393,11 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \
                 ord, p0, shift, partial
+  // sampled timing covers the plain operator D (sign +1, no epilogue) only
   const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->prof_every > 0 && (c->prof_launch[pi]++ % c->prof_every) == 0) {
+  if (c->prof_every > 0 && sign == 1 && p0 == nullptr &&
+      (c->prof_launch[pi]++ % c->prof_every) == 0) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, c->s));
@@ -422,8 +424,9 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   return 0;
 }
 
-// Sampled timing of the operator kernel (k_dslash) launches on the context's
-// stream.  prof_every = 0 switches sampling off.
+// Sampled timing of the operator kernel on the context's stream: every
+// prof_every-th launch of D (sign +1, no epilogue) is bracketed by events.
+// prof_every = 0 switches sampling off.
 extern "C" int susy_prof_enable(susy_ctx *c, int every) {
   if (!c || every < 0) return SUSY_E_ARG;
   HIPCHK(hipStreamSynchronize(c->s));
@@ -435,7 +438,7 @@ extern "C" int susy_prof_enable(susy_ctx *c, int every) {
   c->prof_every = every;
   return 0;
 }
-// Average duration (ms) of the sampled launches and the total launch count,
+// Average duration (ms) of the sampled launches and the count of D launches,
 // for FP64 (prec 0) or FP32 (prec 1) operator kernels.
 extern "C" int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
                               long *sampled) {
