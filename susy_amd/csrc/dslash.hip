@@ -138,10 +138,10 @@ template <typename R, bool DAG>
 __global__ void __launch_bounds__(256)
 k_dslash_prep(const typename CT<R>::t *__restrict__ src,
               const typename CT<R>::t *__restrict__ Uinv,
-              typename CT<R>::t *__restrict__ aux, int V) {
+              typename CT<R>::t *__restrict__ aux, int V, const int *__restrict__ skip) {
   typedef typename CT<R>::t C;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= V) return;
+  if (s >= V || (skip != nullptr && *skip)) return;
   C tr = mkc<C>(0, 0);
 #pragma unroll
   for (int i = 0; i < NCOL; i++) {
@@ -169,19 +169,32 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
   }
 }
 
+#define DS_THREADS (DS_SITES * DS_SPLIT)
+
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
+//
+// A block covers DS_SITES sites of one output component.  With DS_SPLIT = 2
+// two threads share each output matrix: the first computes the site terms
+// a < 3, the first three Dminus terms, or Dplus and the first PtoP pair; the
+// second the rest and the G terms.  Their partial sums meet in LDS and the
+// first stores.  Fewer sites are then in flight per XCD at the same
+// occupancy, so their neighbourhoods compete less for the XCD's L2.
 template <typename R, bool DAG, int EPI>
-__global__ void __launch_bounds__(DS_BLOCK) __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
+__global__ void __launch_bounds__(DS_THREADS) __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
          const int *__restrict__ nbr, const int *__restrict__ tco, int V, int nt,
          R pbc, int doG, R G, DsOrder ord,
          const typename CT<R>::t *__restrict__ p0,
-         R shift, double *__restrict__ partial) {
+         R shift, double *__restrict__ partial, const int *__restrict__ skip) {
   typedef typename CT<R>::t C;
+  // a solver that has converged leaves its remaining queued launches idle
+  // (the flag is one word read by every thread: the exit is block-uniform)
+  if (skip != nullptr && *skip) return;
   const int L = blockIdx.x;
+  const int lane = threadIdx.x % DS_SITES, w = threadIdx.x / DS_SITES;
   const int q = L >> 3;                       // position in this XCD's queue
   const int comp = q % NTF;
   const int qq = q / NTF;                     // chunk position in the queue
@@ -190,12 +203,12 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   if (ord.paired) {
     // even chunk k, odd chunk k, even chunk k+1, ... of this XCD's slab
     const int par = qq & 1, chunk = (L & 7) * ord.slab + (qq >> 1);
-    const int i = chunk * DS_BLOCK + threadIdx.x;
+    const int i = chunk * DS_SITES + lane;
     s = par ? ord.half + i : i;
     live = i < (par ? V - ord.half : ord.half);
   } else {
     const int chunk = (L & 7) * ord.slab + qq;
-    s = chunk * DS_BLOCK + threadIdx.x;
+    s = chunk * DS_SITES + lane;
     live = s < V;
   }
 
@@ -208,25 +221,27 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   auto S = [&](int k, int x) { return Opnd{rS, (uint32_t)x * cs, (uint32_t)k * Mb}; };
   auto Ul = [&](int a, int x) { return Opnd{rU, (uint32_t)x * cs, (uint32_t)a * Mb}; };
 
-  double dot = 0.0;
+  C acc[NCOL][NCOL];
+#pragma unroll
+  for (int i = 0; i < NCOL; i++)
+#pragma unroll
+    for (int k = 0; k < NCOL; k++) acc[i][k] = mkc<C>(0, 0);
+  const bool first = DS_SPLIT == 1 || w == 0, last = DS_SPLIT == 1 || w == DS_SPLIT - 1;
+
   if (live) {
-    C acc[NCOL][NCOL];
-#pragma unroll
-    for (int i = 0; i < NCOL; i++)
-#pragma unroll
-      for (int k = 0; k < NCOL; k++) acc[i][k] = mkc<C>(0, 0);
     const int t = tco[s];
 
     if (comp == 0) {
       // ---- site output: 1/2 sum_a [psi_a(x) Ubar_a(x) - bc(-a) Ubar_a(x-a) psi_a(x-a)]
+      const int a0 = first ? 0 : 3, a1 = last ? NUMLINK : 3;
       DS_UNROLL_LOOP(DS_UNROLL)
-      for (int a = 0; a < NUMLINK; a++) {
+      for (int a = a0; a < a1; a++) {
         const int xm = nb(g_bwd(a));
         mmacc<R, DAG, true>(acc, (R)0.5, S(1 + a, s), Ul(a, s), pl);
         mmacc<R, true, DAG>(acc, (R)-0.5 * bcsign<R>(t, -link_tau(a), nt, pbc), Ul(a, xm),
                             S(1 + a, xm), pl);
       }
-      if (doG) {
+      if (doG && last) {
         // detLtoS: -(i G C2 sqrt(N)/2) sum_{a!=b} plaqdet_ab [T_b(x) + bc(b) T_a(x+b)]
         //          * Lambda_{DIMF-1},  Lambda_{DIMF-1} = i I / sqrt(N)
         const Rsrc rA = make_rsrc(aux, 6 * pl), rP = make_rsrc(pdet, NUMLINK * NUMLINK * pl);
@@ -261,6 +276,8 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       DS_UNROLL_LOOP(DS_UNROLL)
       for (int a = 0; a < NUMLINK; a++) {
         if (a == b) continue;
+        // the first thread takes the first three a != b, the other the last
+        if (DS_SPLIT == 2 && ((a - (a > b)) < 3) != first) continue;
         // Dminus: U_a(x+b) chi_ab(x) - bc(-a) chi_ab(x-a) U_a(x-a)
         const int xm = nb(g_bwd(a));
         const R sg = a < b ? (R)1 : (R)-1;   // chi_ab = sg * stored chi_{min,max}
@@ -269,11 +286,13 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
         mmacc<R, DAG, false>(acc, -sg * bcsign<R>(t, -link_tau(a), nt, pbc), S(pc, xm),
                              Ul(a, xm), pl);
       }
-      // DbplusStoL: 1/2 [bc(b) eta(x+b) Ubar_b(x) - Ubar_b(x) eta(x)]
-      mmacc<R, DAG, true>(acc, (R)0.5 * bcsign<R>(t, link_tau(b), nt, pbc), S(0, xb),
-                          Ul(b, s), pl);
-      mmacc<R, true, DAG>(acc, (R)-0.5, Ul(b, s), S(0, s), pl);
-      if (doG) {
+      if (last) {
+        // DbplusStoL: 1/2 [bc(b) eta(x+b) Ubar_b(x) - Ubar_b(x) eta(x)]
+        mmacc<R, DAG, true>(acc, (R)0.5 * bcsign<R>(t, link_tau(b), nt, pbc), S(0, xb),
+                            Ul(b, s), pl);
+        mmacc<R, true, DAG>(acc, (R)-0.5, Ul(b, s), S(0, s), pl);
+      }
+      if (doG && last) {
         // detStoL: -G C2/2 U_b^-1(x) sum_{c!=b} [Tr eta(x) plaqdet_cb(x)
         //                                        + bc(-c) Tr eta(x-c) plaqdet_bc(x-c)]
         const Rsrc rA = make_rsrc(aux, 6 * pl), rP = make_rsrc(pdet, NUMLINK * NUMLINK * pl);
@@ -309,10 +328,15 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       const int xa = nb(g_fwd(a)), xb = nb(g_fwd(b));
       // Dplus: bc(a) U_a(x) psi_b(x+a) - psi_b(x) U_a(x+b)
       //        - bc(b) U_b(x) psi_a(x+b) + psi_a(x) U_b(x+a)
-      mmacc<R, false, DAG>(acc, bcsign<R>(t, link_tau(a), nt, pbc), Ul(a, s), S(1 + b, xa), pl);
-      mmacc<R, DAG, false>(acc, (R)-1, S(1 + b, s), Ul(a, xb), pl);
-      mmacc<R, false, DAG>(acc, -bcsign<R>(t, link_tau(b), nt, pbc), Ul(b, s), S(1 + a, xb), pl);
-      mmacc<R, DAG, false>(acc, (R)1, S(1 + a, s), Ul(b, xa), pl);
+      if (first) {
+        mmacc<R, false, DAG>(acc, bcsign<R>(t, link_tau(a), nt, pbc), Ul(a, s), S(1 + b, xa),
+                             pl);
+        mmacc<R, DAG, false>(acc, (R)-1, S(1 + b, s), Ul(a, xb), pl);
+        mmacc<R, false, DAG>(acc, -bcsign<R>(t, link_tau(b), nt, pbc), Ul(b, s), S(1 + a, xb),
+                             pl);
+        mmacc<R, DAG, false>(acc, (R)1, S(1 + a, s), Ul(b, xa), pl);
+      }
+      const int n0 = first ? 0 : 1, n1 = last ? 3 : 1;
       // DbplusPtoP + DbminusPtoP.  Each DbminusPtoP term landing on (a,b)
       // coincides with the DbplusPtoP term of the same c: with (d,e) the
       // other two links, x-d-e = x+a+b+c and x-d-e-c = x+a+b (the five e_a
@@ -320,7 +344,7 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
       // offsets agree for the same reason.  The DbplusPtoP terms are applied
       // at double weight (checked term by term when the tables are built).
       DS_UNROLL_LOOP(DS_UNROLL)
-      for (int n = 0; n < 3; n++) {
+      for (int n = n0; n < n1; n++) {
         const int c = c_pt.pp_c[p][n];
         const int pc = 6 + c_pt.pp_de[p][n];
         const int x1 = nb(c_pt.pp_g1[p][n]), x2 = nb(c_pt.pp_g2[p][n]), xmc = nb(g_bwd(c));
@@ -331,7 +355,34 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
                             S(pc, x1), pl);
       }
     }
+  }
 
+#if DS_SPLIT > 1
+  // the second thread's partial sum meets the first's in LDS
+  __shared__ C part[DS_SPLIT - 1][NN][DS_SITES];
+  if (!first) {
+#pragma unroll
+    for (int i = 0; i < NCOL; i++)
+#pragma unroll
+      for (int k = 0; k < NCOL; k++) part[w - 1][i * NCOL + k][lane] = acc[i][k];
+  }
+  __syncthreads();
+  if (first) {
+#pragma unroll
+    for (int u = 0; u < DS_SPLIT - 1; u++)
+#pragma unroll
+      for (int i = 0; i < NCOL; i++)
+#pragma unroll
+        for (int k = 0; k < NCOL; k++) {
+          const C v = part[u][i * NCOL + k][lane];
+          acc[i][k].x += v.x;
+          acc[i][k].y += v.y;
+        }
+  }
+#endif
+
+  double dot = 0.0;
+  if (live && first) {
     // ---- store (D^dagger: negated adjoint)
     const Rsrc rD = make_rsrc(dst, NTF * Mb);
     const Rsrc rQ = make_rsrc(EPI == 1 ? p0 : dst, NTF * Mb);
@@ -353,12 +404,12 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   if (EPI == 1) {
     // deterministic block reduction: wavefront butterfly, then across waves
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
-    __shared__ double red[DS_BLOCK / 64];
+    __shared__ double red[DS_THREADS / 64];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
     __syncthreads();
     if (threadIdx.x == 0) {
       double tsum = 0;
-      for (int w = 0; w < DS_BLOCK / 64; w++) tsum += red[w];
+      for (int u = 0; u < DS_THREADS / 64; u++) tsum += red[u];
       partial[L] = tsum;
     }
   }
@@ -369,7 +420,8 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
                          typename CT<R>::t *dst, int sign, const typename CT<R>::t *p0,
                          R shift, double *partial, int *nblocks,
                          const typename CT<R>::t *U, const typename CT<R>::t *Uinv,
-                         const typename CT<R>::t *pdet, typename CT<R>::t *aux) {
+                         const typename CT<R>::t *pdet, typename CT<R>::t *aux,
+                         const int *skip) {
   if (sign != 1 && sign != -1) {
     set_error("fermion_op: incorrect sign %d", sign);
     return SUSY_E_ARG;
@@ -382,9 +434,11 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   if (c->doG) {
     int nb = (V + 255) / 256;
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash_prep<R, false>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V);
+      hipLaunchKernelGGL((k_dslash_prep<R, false>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V,
+                         skip);
     else
-      hipLaunchKernelGGL((k_dslash_prep<R, true>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V);
+      hipLaunchKernelGGL((k_dslash_prep<R, true>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V,
+                         skip);
     HIPCHK(hipGetLastError());
   }
   DsOrder ord;
@@ -392,7 +446,7 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \
-                ord, p0, shift, partial
+                ord, p0, shift, partial, skip
   // sampled timing covers the plain operator D (sign +1, no epilogue) only
   const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -405,14 +459,14 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   const unsigned lds = sizeof(R) == sizeof(double) ? DS_OCC_LDS64 : DS_OCC_LDS32;
   if (p0 == nullptr) {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
   } else {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(DS_BLOCK), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
   }
 #undef DS_ARGS
   HIPCHK(hipGetLastError());
@@ -458,17 +512,16 @@ extern "C" int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launc
 }
 
 int dslash64(susy_ctx *c, const double2 *src, double2 *dst, int sign,
-             const double2 *p0, double shift, double *partial, int *nblocks) {
+             const double2 *p0, double shift, double *partial, int *nblocks, const int *skip) {
   return dslash_launch<double>(c, src, dst, sign, p0, shift, partial, nblocks,
-                               c->d_links, c->d_uinv, c->d_pdet, c->d_aux);
+                               c->d_links, c->d_uinv, c->d_pdet, c->d_aux, skip);
 }
-
 int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
-             const float2 *p0, float shift, double *partial, int *nblocks) {
+             const float2 *p0, float shift, double *partial, int *nblocks, const int *skip) {
   if (!c->f_fresh) {
     int r = fields_refresh32(c);
     if (r) return r;
   }
   return dslash_launch<float>(c, src, dst, sign, p0, shift, partial, nblocks,
-                              c->f_links, c->f_uinv, c->f_pdet, c->f_aux);
+                              c->f_links, c->f_uinv, c->f_pdet, c->f_aux, skip);
 }

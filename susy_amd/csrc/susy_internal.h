@@ -25,9 +25,13 @@
 #define NTERMS 30
 #define NQLINK 28
 #define IMAG_TOL 1.0e-8
-// sites per block of the fused operator kernel
-#ifndef DS_BLOCK
-#define DS_BLOCK 128
+// operator kernel: a block covers DS_SITES sites of one output component,
+// with DS_SPLIT wavefronts sharing the products of each output matrix
+#ifndef DS_SITES
+#define DS_SITES 128
+#endif
+#ifndef DS_SPLIT
+#define DS_SPLIT 1
 #endif
 #define C2 1.0
 
@@ -47,12 +51,12 @@ inline int ds_grid(int V, int paired, DsOrder *o) {
   d.paired = paired;
   d.half = V / 2;
   if (paired) {
-    const int chunks = (V - d.half + DS_BLOCK - 1) / DS_BLOCK;   // odd half is the larger
+    const int chunks = (V - d.half + DS_SITES - 1) / DS_SITES;   // odd half is the larger
     d.slab = (chunks + 7) / 8;
     if (o) *o = d;
     return 8 * d.slab * 2 * NTF;
   }
-  const int chunks = (V + DS_BLOCK - 1) / DS_BLOCK;
+  const int chunks = (V + DS_SITES - 1) / DS_SITES;
   d.slab = (chunks + 7) / 8;
   if (o) *o = d;
   return 8 * d.slab * NTF;
@@ -218,10 +222,13 @@ void set_error(const char *fmt, ...);
 int dslash_init_tables(const PlaqTerms &pt);
 // dst = D src (sign +1) or D^dag src (sign -1).  If p0 != nullptr the
 // epilogue writes dst + shift*p0 and a per-block partial of Re<p0, dst>.
+// skip: optional device flag; when it reads non-zero the launches do nothing
 int dslash64(susy_ctx *c, const double2 *src, double2 *dst, int sign,
-             const double2 *p0, double shift, double *partial, int *nblocks);
+             const double2 *p0, double shift, double *partial, int *nblocks,
+             const int *skip = nullptr);
 int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
-             const float2 *p0, float shift, double *partial, int *nblocks);
+             const float2 *p0, float shift, double *partial, int *nblocks,
+             const int *skip = nullptr);
 
 int fields_recompute(susy_ctx *c);      // det, Uinv, plaqdet, ZW*, DmuUmu, Fmunu
 int fields_refresh32(susy_ctx *c);      // FP32 copies of links, Uinv, plaqdet
