@@ -96,14 +96,13 @@ static void free_ctx(susy_ctx *c) {
                   c->d_uinv, c->d_det, c->d_pdet, c->d_zw, c->d_dmu, c->d_fmunu,
                   c->d_force, c->f_links, c->f_uinv, c->f_pdet, c->d_aux, c->f_aux,
                   c->d_tmp, c->f_tmp, c->d_boundary, c->d_part, c->d_scal, c->cg_ws,
-                  c->cg_t64, c->d_coef, c->f_in, c->f_out};
+                  c->cg_t64, c->cg_state, c->f_in, c->f_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double2 *p : c->tf)
     if (p) (void)hipFree(p);
   for (double2 *p : c->traj_pool)
     if (p) (void)hipFree(p);
-  if (c->h_coef) (void)hipHostFree(c->h_coef);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;

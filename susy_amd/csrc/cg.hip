@@ -31,6 +31,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "susy_internal.h"
 
 #define CB 256
@@ -85,13 +87,132 @@ k_cg_init(const double2 *__restrict__ src, C *__restrict__ r, C *__restrict__ p0
   cg_block_sum(s, partial);
 }
 
-// rout = rin + beta0 * ap; partial |rout|^2.  rout is the next slot of the
+// Scalar state of one solve, in device memory.  The reference's recurrences
+// (congrad_multi.c:60-180) run in single-thread kernels between the vector
+// passes, so the host enqueues iterations without waiting for them; a
+// solver that has converged turns every queued launch into a no-op.
+struct CgState {
+  int done, iteration;          // read back by the host (first 24 bytes)
+  double rsq, rsqstop;
+  int skip_rel, maxcg, nsh, cnt;
+  double shifts[MAXSH];
+  double zi[MAXSH], zim1[MAXSH], zip1[MAXSH], bi[MAXSH], bim1[MAXSH], al[MAXSH];
+  int conv[MAXSH], touched[MAXSH];
+  double A[MAXSH], E[MAXSH], Cw[MAXSH][MAXWIN], Dw[MAXSH][MAXWIN];
+};
+struct Shifts {
+  double v[MAXSH];
+};
+
+// Arithmetic of the scalar kernels is not contracted into FMAs, so it rounds
+// like the reference's host code.
+__global__ void k_cgs_init(CgState *st, const double *__restrict__ source_norm,
+                           double errormin, int nsh, Shifts sh, int maxcg) {
+#pragma clang fp contract(off)
+  st->rsq = *source_norm;
+  st->rsqstop = errormin * st->rsq;
+  st->iteration = 0;
+  st->maxcg = maxcg;
+  st->nsh = nsh;
+  st->cnt = 0;
+  st->done = !(0 < maxcg && st->rsq > st->rsqstop);
+  st->skip_rel = 0;
+  for (int j = 0; j < MAXSH; j++) {
+    st->shifts[j] = j < nsh ? sh.v[j] : 0.0;
+    st->zim1[j] = 1; st->zi[j] = 1; st->al[j] = 0; st->bim1[j] = 1; st->conv[j] = 0;
+    st->zip1[j] = 1; st->bi[j] = 0;
+    st->A[j] = 1; st->E[j] = 0; st->touched[j] = 0;
+    for (int l = 0; l < MAXWIN; l++) st->Cw[j][l] = st->Dw[j][l] = 0;
+  }
+}
+
+// the while condition of the iteration loop, evaluated at the end of each
+// iteration (after its p update, which the converged iteration still makes):
+// everything queued after it -- flushes, reliable updates, further
+// iterations -- then sees the solve finished
+__global__ void k_cgs_stop(CgState *st) {
+  if (!st->done && !(st->iteration < st->maxcg && st->rsq > st->rsqstop)) st->done = 1;
+}
+
+// beta and the shifted zeta / beta after A p0 and <p0, A p0> = *cd
+__global__ void k_cgs_beta(CgState *st, const double *__restrict__ cd) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  st->iteration++;
+  double *zi = st->zi, *zim1 = st->zim1, *zip1 = st->zip1, *bi = st->bi, *bim1 = st->bim1,
+         *al = st->al;
+  const double *shifts = st->shifts;
+  bi[0] = -st->rsq / *cd;
+  zip1[0] = 1;
+  for (int j = 1; j < st->nsh; j++) {
+    if (st->conv[j]) continue;
+    zip1[j] = zi[j] * zim1[j] * bim1[0];
+    double c1 = bi[0] * al[0] * (zim1[j] - zi[j]);
+    double c2 = zim1[j] * bim1[0] * (1 - (shifts[j] - shifts[0]) * bi[0]);
+    zip1[j] /= c1 + c2;
+    bi[j] = bi[0] * zip1[j] / zi[j];
+  }
+}
+
+// alpha, the window coefficients, convergence of the shifted systems and the
+// advance of the recurrences after |r_new|^2 = *rsqnew
+__global__ void k_cgs_alpha(CgState *st, const double *__restrict__ rsqnew) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  double *zi = st->zi, *zim1 = st->zim1, *zip1 = st->zip1, *bi = st->bi, *bim1 = st->bim1,
+         *al = st->al;
+  const int nsh = st->nsh, cnt = st->cnt;
+  al[0] = *rsqnew / st->rsq;
+  for (int j = 1; j < nsh; j++)
+    if (!st->conv[j]) al[j] = al[0] * zip1[j] * bi[j] / (zi[j] * bi[0]);
+  // psi_j -= beta_j p_j, then p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
+  for (int j = 0; j < nsh; j++) {
+    if (st->conv[j]) continue;
+    st->touched[j] = 1;
+    st->E[j] -= bi[j] * st->A[j];
+    for (int l = 0; l < cnt; l++) st->Dw[j][l] -= bi[j] * st->Cw[j][l];
+    st->A[j] *= al[j];
+    for (int l = 0; l < cnt; l++) st->Cw[j][l] *= al[j];
+    st->Cw[j][cnt] = zip1[j];
+  }
+  st->cnt = cnt + 1;
+  st->rsq = *rsqnew;
+  for (int j = 1; j < nsh; j++)
+    if (!st->conv[j] && st->rsq * zip1[j] * zip1[j] <= st->rsqstop) st->conv[j] = 1;
+  for (int j = 0; j < nsh; j++)
+    if (!st->conv[j]) { bim1[j] = bi[j]; zim1[j] = zi[j]; zi[j] = zip1[j]; }
+}
+
+// window state after a flush
+__global__ void k_cgs_reset(CgState *st, const int *__restrict__ skip) {
+  if ((skip != nullptr && *skip) || st->cnt == 0) return;
+  for (int j = 0; j < MAXSH; j++) {
+    st->A[j] = 1; st->E[j] = 0; st->touched[j] = 0;
+    for (int l = 0; l < MAXWIN; l++) st->Cw[j][l] = st->Dw[j][l] = 0;
+  }
+  st->cnt = 0;
+}
+
+// the reliable update runs only while the solve is open and unconverged
+__global__ void k_cgs_relcheck(CgState *st) {
+  st->skip_rel = st->done || !(st->rsq > st->rsqstop);
+}
+__global__ void k_cgs_setrsq(CgState *st, const double *__restrict__ v,
+                             const int *__restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  st->rsq = *v;
+}
+__global__ void k_cgs_reopen(CgState *st) { st->done = 0; }
+
+// rout = rin + beta_0 * ap; partial |rout|^2.  rout is the next slot of the
 // residual window and rin the previous one (the same slot when the window
 // holds one residual: the update is elementwise, so in place is safe).
 template <typename C>
 __global__ void __launch_bounds__(CB)
-k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, double beta0, size_t n,
-       double *__restrict__ partial) {
+k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, const CgState *__restrict__ st,
+       size_t n, double *__restrict__ partial) {
+  if (st->done) return;
+  const double beta0 = st->bi[0];
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
     C a = ap[i], v = rin[i];
@@ -103,55 +224,54 @@ k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, double beta0, size_t n,
   cg_block_sum(s, partial);
 }
 
-// pout = r + alpha0 pin (pout is pin, or the other p0 buffer at the first
+// pout = r + alpha_0 pin (pout is pin, or the other p0 buffer at the first
 // iteration of a window, which leaves the window-start p0 intact)
 template <typename C>
 __global__ void __launch_bounds__(CB)
-k_cg_p0(const C *pin, const C *__restrict__ r, C *pout, double alpha0, size_t n) {
+k_cg_p0(const C *pin, const C *__restrict__ r, C *pout, const CgState *__restrict__ st,
+        size_t n) {
+  if (st->done) return;
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
+  const double alpha0 = st->al[0];
   C p = pin[i], rv = r[i];
   p.x = rv.x + alpha0 * p.x;
   p.y = rv.y + alpha0 * p.y;
   pout[i] = p;
 }
 
-// Window flush.  Slot k updates psi_j (j = sh[k]) and, for j >= 1, p_j:
-//   p_j   <- A p_j(k0) + sum_l C_l r_l
-//   psi_j <- psi_j + E p_j(k0) + sum_l D_l r_l
-// coef layout per slot: [A, E, C_0..C_{MAXWIN-1}, D_0..D_{MAXWIN-1}].  For the
-// base system (j = 0) p(k0) is the frozen window-start p0 and is not written.
-struct FlushArgs {
-  int nact;
-  int sh[MAXSH];
-};
+// Window flush.  Every touched system j updates psi_j and, for j >= 1, p_j:
+//   p_j   <- A_j p_j(k0) + sum_l C_jl r_l
+//   psi_j <- psi_j + E_j p_j(k0) + sum_l D_jl r_l
+// with the coefficients of the device state; pp.p[j] is p_j(k0) (for the
+// base system the frozen window-start p0, which is not written).
 struct PPtrs {
   void *p[MAXSH];
 };
 template <typename C, int M>
 __global__ void __launch_bounds__(CB)
-k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win,
-           const double *__restrict__ coef, FlushArgs fa, size_t n) {
+k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__restrict__ st,
+           const int *__restrict__ skip, size_t n) {
+  if ((skip != nullptr && *skip) || st->cnt == 0) return;
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
   // all M slots are read; slots beyond this window's count carry zero weights
   double2 w[M];
 #pragma unroll
   for (int l = 0; l < M; l++) w[l] = to64(win[(size_t)l * n + i]);
-  const int stride = 2 + 2 * MAXWIN;
-  for (int k = 0; k < fa.nact; k++) {
-    const int j = fa.sh[k];
-    const double *cf = coef + k * stride;
-    C *pk = (C *)pp.p[k] + i;
+  const int nsh = st->nsh;
+  for (int j = 0; j < nsh; j++) {
+    if (!st->touched[j]) continue;
+    C *pk = (C *)pp.p[j] + i;
     double2 p = to64(*pk);
     double2 x = psi.p[j][i];
-    double A = cf[0], E = cf[1];
+    const double A = st->A[j], E = st->E[j];
     double2 np = make_double2(A * p.x, A * p.y);
     x.x += E * p.x;
     x.y += E * p.y;
 #pragma unroll
     for (int l = 0; l < M; l++) {
-      double cl = cf[2 + l], dl = cf[2 + MAXWIN + l];
+      const double cl = st->Cw[j][l], dl = st->Dw[j][l];
       np.x += cl * w[l].x;
       np.y += cl * w[l].y;
       x.x += dl * w[l].x;
@@ -166,7 +286,9 @@ k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win,
 template <typename C>
 __global__ void __launch_bounds__(CB)
 k_true_res(const double2 *__restrict__ src, const double2 *__restrict__ ap,
-           C *__restrict__ r, size_t n, double *__restrict__ partial) {
+           C *__restrict__ r, size_t n, double *__restrict__ partial,
+           const int *__restrict__ skip) {
+  if (skip != nullptr && *skip) return;
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
     double2 a = src[i], b = ap[i];
@@ -197,6 +319,9 @@ extern "C" int susy_set_cg_options(int window, int reliable) {
   return 0;
 }
 
+// iterations enqueued between two read-backs of the solver status
+#define CG_BATCH 16
+
 template <typename R>
 static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int nsh,
                    const double *shifts, int MaxCG, double errormin, double *size_r) {
@@ -217,190 +342,180 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
     HIPCHK(hipMemsetAsync(c->cg_ws, 0, need, c->s));
     c->cg_ws_bytes = need;
   }
+  if (!c->cg_state) HIPCHK(hipMalloc(&c->cg_state, sizeof(CgState)));
+  CgState *st = (CgState *)c->cg_state;
+  const int *d_done = &st->done, *d_skiprel = &st->skip_rel;
   C *base = (C *)c->cg_ws;
   C *p0buf[2] = {base, base + n};
   C *ap = base + 2 * n, *tmp = base + 3 * n;
   C *pj = base + 4 * n;
   C *win = pj + (size_t)(nsh - 1) * n;
   auto slot = [&](int l) { return win + (size_t)l * n; };
-  // r is the newest window slot; before the first iteration the last one
-  C *r = slot(m - 1);
-  // p0: pcur is the current direction, pfrozen its value at window start
-  C *pcur = p0buf[0], *pfrozen = p0buf[0];
   double2 *ap64 = nullptr, *tmp64 = nullptr;
   if (fp32) {
     if (!c->cg_t64) HIPCHK(hipMalloc(&c->cg_t64, 2 * n * sizeof(double2)));
     ap64 = c->cg_t64;
     tmp64 = c->cg_t64 + n;
   }
-
   PsiPtrs psi;
   for (int j = 0; j < nsh; j++) psi.p[j] = psim[j];
   double sigma0 = shifts[0] + ((c->p.fmass > IMAG_TOL) ? c->p.fmass * c->p.fmass : 0.0);
+  double *d_norm = c->d_scal, *d_cd = c->d_scal + 1, *d_rn = c->d_scal + 2,
+         *d_true = c->d_scal + 3;
 
-  int nb = nblk(n);
+  // Host bookkeeping of the buffers, a function of the iteration count: r is
+  // the newest window slot (before the first iteration the last one), pcur
+  // the current p0 and pfrozen its value at the window start.
+  C *r = slot(m - 1), *pcur = p0buf[0], *pfrozen = p0buf[0];
+  int host_cnt = 0, since_reliable = 0;
+
+  const int nb = nblk(n);
   hipLaunchKernelGGL((k_cg_init<C>), dim3(nb), dim3(CB), 0, c->s, src, r, pcur, pj, nsh, psi,
                      n, c->d_part);
   HIPCHK(hipGetLastError());
   int e = reduce_partials(c, c->d_part, nb, 0);
   if (e) return e;
-  double source_norm;
-  if ((e = read_scalars(c, &source_norm, 1))) return e;
-  double rsq = source_norm, rsqstop = errormin * source_norm;
+  Shifts shv;
+  for (int j = 0; j < MAXSH; j++) shv.v[j] = j < nsh ? shifts[j] : 0.0;
+  hipLaunchKernelGGL(k_cgs_init, dim3(1), dim3(1), 0, c->s, st, d_norm, errormin, nsh, shv,
+                     MaxCG);
+  HIPCHK(hipGetLastError());
 
-  double zi[MAXSH], zim1[MAXSH], zip1[MAXSH], bi[MAXSH], bim1[MAXSH], al[MAXSH];
-  int conv[MAXSH];
-  for (int j = 0; j < nsh; j++) {
-    zim1[j] = 1; zi[j] = 1; al[j] = 0; bim1[j] = 1; conv[j] = 0; zip1[j] = 1; bi[j] = 0;
-  }
-  // window coefficients, per system j (0 = base): A, E, C[m], D[m]
-  const int stride = 2 + 2 * MAXWIN;
-  if (!c->h_coef) {
-    HIPCHK(hipHostMalloc((void **)&c->h_coef, sizeof(double) * stride * MAXSH));
-    HIPCHK(hipMalloc((void **)&c->d_coef, sizeof(double) * stride * MAXSH));
-  }
-  double *h_coef = c->h_coef, *d_coef = c->d_coef;
-  double A[MAXSH], E[MAXSH], Cw[MAXSH][MAXWIN], Dw[MAXSH][MAXWIN];
-  int touched[MAXSH];
-  int cnt = 0;
-  auto reset_window = [&]() {
-    for (int j = 0; j < nsh; j++) {
-      A[j] = 1; E[j] = 0; touched[j] = 0;
-      for (int l = 0; l < MAXWIN; l++) Cw[j][l] = Dw[j][l] = 0;
-    }
-    cnt = 0;
-    pfrozen = pcur;
-  };
-  auto flush = [&]() -> int {
-    FlushArgs fa;
-    PPtrs pp;
-    fa.nact = 0;
-    for (int j = 0; j < nsh; j++) {
-      if (!touched[j]) continue;
-      double *cf = h_coef + fa.nact * stride;
-      cf[0] = A[j]; cf[1] = E[j];
-      for (int l = 0; l < MAXWIN; l++) { cf[2 + l] = Cw[j][l]; cf[2 + MAXWIN + l] = Dw[j][l]; }
-      pp.p[fa.nact] = j == 0 ? (void *)pfrozen : (void *)(pj + (size_t)(j - 1) * n);
-      fa.sh[fa.nact++] = j;
-    }
-    if (fa.nact > 0) {
-      HIPCHK(hipMemcpyAsync(d_coef, h_coef, sizeof(double) * stride * fa.nact,
-                            hipMemcpyHostToDevice, c->s));
-      dim3 g((n + CB - 1) / CB), b(CB);
-      switch (m) {
-        case 1: hipLaunchKernelGGL((k_cg_flush<C, 1>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
-        case 4: hipLaunchKernelGGL((k_cg_flush<C, 4>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
-        case 8: hipLaunchKernelGGL((k_cg_flush<C, 8>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
-        case 16: hipLaunchKernelGGL((k_cg_flush<C, 16>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
-        default: hipLaunchKernelGGL((k_cg_flush<C, 32>), g, b, 0, c->s, pp, psi, win, d_coef, fa, n); break;
-      }
-      HIPCHK(hipGetLastError());
-      // the pinned coefficient buffer is reused by the next flush
-      HIPCHK(hipStreamSynchronize(c->s));
-    }
-    reset_window();
+  auto one = [&](void (*k)(CgState *)) -> int {
+    hipLaunchKernelGGL(k, dim3(1), dim3(1), 0, c->s, st);
+    HIPCHK(hipGetLastError());
     return 0;
   };
-  reset_window();
-
-  int iteration = 0, since_reliable = 0;
-  auto dsq = [&](const C *in, C *out, double *part, int *nbo) -> int {
+  // flush the window (if the device's window holds any iteration) and reset it
+  auto flush = [&](const int *skip) -> int {
+    PPtrs pp;
+    pp.p[0] = pfrozen;
+    for (int j = 1; j < nsh; j++) pp.p[j] = pj + (size_t)(j - 1) * n;
+    dim3 g((n + CB - 1) / CB), b(CB);
+    switch (m) {
+      case 1: hipLaunchKernelGGL((k_cg_flush<C, 1>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 4: hipLaunchKernelGGL((k_cg_flush<C, 4>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 8: hipLaunchKernelGGL((k_cg_flush<C, 8>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 16: hipLaunchKernelGGL((k_cg_flush<C, 16>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      default: hipLaunchKernelGGL((k_cg_flush<C, 32>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+    }
+    hipLaunchKernelGGL(k_cgs_reset, dim3(1), dim3(1), 0, c->s, st, skip);
+    HIPCHK(hipGetLastError());
+    host_cnt = 0;
+    pfrozen = pcur;
+    return 0;
+  };
+  auto dsq = [&](const C *in, C *out, int *nbo) -> int {
     int r1, nbd = 0;
     if (fp32) {
-      r1 = dslash32(c, (const float2 *)in, (float2 *)tmp, 1, nullptr, 0.f, nullptr, nullptr);
+      r1 = dslash32(c, (const float2 *)in, (float2 *)tmp, 1, nullptr, 0.f, nullptr, nullptr,
+                    d_done);
       if (!r1) r1 = dslash32(c, (const float2 *)tmp, (float2 *)out, -1, (const float2 *)in,
-                             (float)sigma0, part, &nbd);
+                             (float)sigma0, c->d_part, &nbd, d_done);
     } else {
-      r1 = dslash64(c, (const double2 *)in, (double2 *)tmp, 1, nullptr, 0.0, nullptr, nullptr);
+      r1 = dslash64(c, (const double2 *)in, (double2 *)tmp, 1, nullptr, 0.0, nullptr, nullptr,
+                    d_done);
       if (!r1) r1 = dslash64(c, (const double2 *)tmp, (double2 *)out, -1, (const double2 *)in,
-                             sigma0, part, &nbd);
+                             sigma0, c->d_part, &nbd, d_done);
     }
     *nbo = nbd;
     return r1;
   };
-  // FP32 form: replace r by the FP64 true residual of the base system
-  auto reliable_update = [&]() -> int {
+  // FP32 form: replace r by the FP64 true residual of the base system (skip:
+  // the device flag that cancels it, or nullptr to run it unconditionally)
+  auto reliable_update = [&](const int *skip) -> int {
     int r1, nbd = 0;
-    if (cnt > 0 && (r1 = flush())) return r1;
-    r1 = dslash64(c, psim[0], tmp64, 1, nullptr, 0.0, nullptr, nullptr);
-    if (!r1) r1 = dslash64(c, tmp64, ap64, -1, psim[0], sigma0, c->d_part, &nbd);
+    if ((r1 = flush(skip))) return r1;
+    r1 = dslash64(c, psim[0], tmp64, 1, nullptr, 0.0, nullptr, nullptr, skip);
+    if (!r1) r1 = dslash64(c, tmp64, ap64, -1, psim[0], sigma0, c->d_part, &nbd, skip);
     if (r1) return r1;
-    hipLaunchKernelGGL((k_true_res<C>), dim3(nb), dim3(CB), 0, c->s, src, ap64, r, n, c->d_part);
+    hipLaunchKernelGGL((k_true_res<C>), dim3(nb), dim3(CB), 0, c->s, src, ap64, r, n, c->d_part,
+                       skip);
     HIPCHK(hipGetLastError());
-    if ((r1 = reduce_partials(c, c->d_part, nb, 0))) return r1;
-    return read_scalars(c, &rsq, 1);
+    if ((r1 = reduce_partials(c, c->d_part, nb, 3))) return r1;
+    hipLaunchKernelGGL(k_cgs_setrsq, dim3(1), dim3(1), 0, c->s, st, d_true, skip);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  struct Status {
+    int done, iteration;
+    double rsq, rsqstop;
+  } stat = {0, 0, 0.0, 0.0};
+  auto read_status = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(c->h_scal, st, sizeof(Status), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    memcpy(&stat, c->h_scal, sizeof(Status));
+    return 0;
   };
 
+  // buffers after each queued iteration (before its flush / reliable update)
+  struct Pos {
+    C *r, *p, *pfrozen;
+  };
+  Pos log[CG_BATCH];
   for (;;) {
-    while (iteration < MaxCG && rsq > rsqstop) {
-      int nbd;
-      if ((e = dsq(pcur, ap, c->d_part, &nbd))) return e;
-      if ((e = reduce_partials(c, c->d_part, nbd, 0))) return e;
-      double cd;
-      if ((e = read_scalars(c, &cd, 1))) return e;
-      iteration++;
-      c->total_iters++;
-      bi[0] = -rsq / cd;
-      zip1[0] = 1;
-      for (int j = 1; j < nsh; j++) {
-        if (conv[j]) continue;
-        zip1[j] = zi[j] * zim1[j] * bim1[0];
-        double c1 = bi[0] * al[0] * (zim1[j] - zi[j]);
-        double c2 = zim1[j] * bim1[0] * (1 - (shifts[j] - shifts[0]) * bi[0]);
-        zip1[j] /= c1 + c2;
-        bi[j] = bi[0] * zip1[j] / zi[j];
+    // batches of iterations until the device reports the loop finished
+    for (;;) {
+      const int it_before = stat.iteration;
+      const Pos start = {r, pcur, pfrozen};
+      for (int k = 0; k < CG_BATCH; k++) {
+        int nbd;
+        if ((e = dsq(pcur, ap, &nbd))) return e;
+        if ((e = reduce_partials(c, c->d_part, nbd, 1))) return e;
+        hipLaunchKernelGGL(k_cgs_beta, dim3(1), dim3(1), 0, c->s, st, d_cd);
+        C *rnew = slot(host_cnt);
+        hipLaunchKernelGGL((k_cg_r<C>), dim3(nb), dim3(CB), 0, c->s, r, ap, rnew, st, n,
+                           c->d_part);
+        HIPCHK(hipGetLastError());
+        if ((e = reduce_partials(c, c->d_part, nb, 2))) return e;
+        hipLaunchKernelGGL(k_cgs_alpha, dim3(1), dim3(1), 0, c->s, st, d_rn);
+        C *pnext = pcur != pfrozen ? pcur : (pcur == p0buf[0] ? p0buf[1] : p0buf[0]);
+        hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, pcur,
+                           rnew, pnext, st, n);
+        HIPCHK(hipGetLastError());
+        r = rnew;
+        pcur = pnext;
+        host_cnt++;
+        if ((e = one(k_cgs_stop))) return e;
+        log[k] = {r, pcur, pfrozen};
+        // window end: a solve that has finished keeps its open window for
+        // the final flush below, made with the restored buffers
+        if (host_cnt == m && (e = flush(d_done))) return e;
+        if (fp32 && ++since_reliable >= g_reliable) {
+          since_reliable = 0;
+          if ((e = one(k_cgs_relcheck))) return e;
+          if ((e = reliable_update(d_skiprel))) return e;
+        }
       }
-      C *rnew = slot(cnt);
-      hipLaunchKernelGGL((k_cg_r<C>), dim3(nb), dim3(CB), 0, c->s, r, ap, rnew, bi[0], n,
-                         c->d_part);
-      HIPCHK(hipGetLastError());
-      r = rnew;
-      if ((e = reduce_partials(c, c->d_part, nb, 0))) return e;
-      double rsqnew;
-      if ((e = read_scalars(c, &rsqnew, 1))) return e;
-      al[0] = rsqnew / rsq;
-      for (int j = 1; j < nsh; j++)
-        if (!conv[j]) al[j] = al[0] * zip1[j] * bi[j] / (zi[j] * bi[0]);
-      C *pnext = pcur != pfrozen ? pcur : (pcur == p0buf[0] ? p0buf[1] : p0buf[0]);
-      hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, pcur, r,
-                         pnext, al[0], n);
-      HIPCHK(hipGetLastError());
-      pcur = pnext;
-      // advance the window coefficients: psi_j -= beta_j p_j, then
-      // p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
-      for (int j = 0; j < nsh; j++) {
-        if (conv[j]) continue;
-        touched[j] = 1;
-        E[j] -= bi[j] * A[j];
-        for (int l = 0; l < cnt; l++) Dw[j][l] -= bi[j] * Cw[j][l];
-        A[j] *= al[j];
-        for (int l = 0; l < cnt; l++) Cw[j][l] *= al[j];
-        Cw[j][cnt] = zip1[j];
-      }
-      cnt++;
-      rsq = rsqnew;
-      for (int j = 1; j < nsh; j++)
-        if (!conv[j] && rsq * zip1[j] * zip1[j] <= rsqstop) conv[j] = 1;
-      for (int j = 0; j < nsh; j++)
-        if (!conv[j]) { bim1[j] = bi[j]; zim1[j] = zi[j]; zi[j] = zip1[j]; }
-      if (cnt == m && (e = flush())) return e;
-      if (fp32 && ++since_reliable >= g_reliable && rsq > rsqstop) {
-        since_reliable = 0;
-        if ((e = reliable_update())) return e;
-      }
+      if ((e = read_status())) return e;
+      if (!stat.done) continue;
+      // The loop ended inside this batch: the launches queued after it did
+      // nothing (flushes and reliable updates included), so the buffers are
+      // those of its last real iteration.
+      const int real = stat.iteration - it_before;
+      const Pos at = real > 0 ? log[real - 1] : start;
+      r = at.r;
+      pcur = at.p;
+      pfrozen = at.pfrozen;
+      break;
     }
-    if (cnt > 0 && (e = flush())) return e;
-    if (!fp32 || iteration >= MaxCG) break;
+    if ((e = flush(nullptr))) return e;
+    if (!fp32 || stat.iteration >= MaxCG) break;
     // FP32 form: confirm convergence on the FP64 true residual
-    double iter_rsq = rsq;
-    if ((e = reliable_update())) return e;
+    if ((e = reliable_update(nullptr))) return e;
     since_reliable = 0;
-    if (rsq <= rsqstop) break;
-    (void)iter_rsq;
+    if ((e = read_status())) return e;
+    // resume only as the loop condition would (a NaN residual ends the solve)
+    if (!(stat.rsq > stat.rsqstop)) break;
+    if ((e = one(k_cgs_reopen))) return e;
+    stat.done = 0;
   }
-  if (rsq > rsqstop) printf(" multi CONGRAD not converged\n rsq = %.4g\n", rsq);
-  *size_r = rsq;
-  return iteration;
+  if ((e = read_status())) return e;
+  c->total_iters += stat.iteration;
+  if (!(stat.rsq <= stat.rsqstop))
+    printf(" multi CONGRAD not converged\n rsq = %.4g\n", stat.rsq);
+  *size_r = stat.rsq;
+  return stat.iteration;
 }
 
 int cg_multi(susy_ctx *c, const double2 *src, double2 *const *psim, int nshift,

@@ -188,7 +188,7 @@ struct susy_ctx {
   void *cg_ws = nullptr;
   size_t cg_ws_bytes = 0;
   double2 *cg_t64 = nullptr;        // two FP64 fields for true residuals
-  double *h_coef = nullptr, *d_coef = nullptr;
+  void *cg_state = nullptr;         // scalar state of the multi-shift CG (device)
   float2 *f_in = nullptr, *f_out = nullptr;   // FP32 operator staging
   std::vector<double2 *> traj_pool;           // pseudofermions + solutions
   // RNG (host, bit-exact with the reference)

@@ -126,6 +126,32 @@ def test_congrad_multi_fp32_form(ncol):
         assert np.vdot(r, r).real <= 4 * err * src_n2, j
 
 
+@pytest.mark.parametrize("offset", [-1, 0, 1])
+def test_congrad_multi_fp32_reliable_at_convergence(offset):
+    """the solver enqueues iterations ahead of its convergence test: a reliable
+    update falling on the converged iteration (or just after it) must leave
+    the solutions intact"""
+    say("cg32 reliable", offset)
+    O, lat = pair(2)
+    src = random_tf(O.tf_shape(), 3)
+    shifts = O.rhmc()[16:31]
+    err = 1e-10
+    it_o = O.congrad_multi(src, shifts, 5000, err)[0]
+    hs = lat.tf_alloc(1 + 15 + 1)
+    lat.tf_upload(hs, src)
+    for window in (32, 16, 1):
+        susy_amd.set_cg_options(2, window=window, reliable=it_o + offset)
+        it, sz = lat.congrad_multi(hs, hs + 1, shifts, 5000, err, form=susy_amd.FORM_FP32)
+        assert it <= 1.2 * it_o + 20, (window, it, it_o)
+        src_n2 = np.vdot(src, src).real
+        for j in (0, 7, 14):
+            lat.dsq(hs + 1 + j, hs + 16)
+            psi = lat.tf_download(hs + 1 + j)
+            r = src - (lat.tf_download(hs + 16) + shifts[j] * psi)
+            assert np.vdot(r, r).real <= 4 * err * src_n2, (window, j)
+    susy_amd.set_cg_options(2, window=32, reliable=50)
+
+
 @pytest.mark.parametrize("ncol", [2, 3, 4])
 def test_forces(ncol):
     say("forces", ncol)
