@@ -213,6 +213,16 @@ int susy_set_site_order(susy_ctx *c, int paired);
    inner (gauge) force and removes it from every outer (fermion) kick, so the
    outer level integrates S_F - S_theta and the exact action still decides
    the Metropolis test. */
+/* Extreme eigenvalues of DSq (D^dag D + fmass^2) at the context's links, the
+   measurement the reference prints in testsuite/scalar/eig.U*.ref (there
+   with PRIMME): Lanczos with full reorthogonalisation on the FP64 operator,
+   stopping when the nlow smallest and nhigh largest Ritz values move by less
+   than tol (relative) over 20 iterations, or after maxiter.  Distinct
+   eigenvalues are resolved; an exact degeneracy shows up only as far as
+   rounding seeds its second direction.  Holds maxiter + 1 fields in HBM. */
+int susy_eig_dsq(susy_ctx *c, int nlow, int nhigh, int maxiter, double tol, int seed,
+                 double *low, double *high, int *iters);
+
 int susy_sg_nparams(void);
 int susy_sg_init(susy_ctx *c, int seed, double scale);
 int susy_sg_set_params(susy_ctx *c, const float *theta);

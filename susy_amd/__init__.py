@@ -49,7 +49,7 @@ EXPORTS = [
     "susy_set_verbose", "susy_stream_copy_dev", "susy_set_site_order",
     "susy_sg_nparams", "susy_sg_init", "susy_sg_set_params", "susy_sg_get_params",
     "susy_sg_eval", "susy_sg_features", "susy_sg_loss_grad", "susy_sg_record", "susy_sg_pairs",
-    "susy_sg_pair_loss", "susy_sg_train",
+    "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq",
 ]
 
 _libs = {}
@@ -107,6 +107,7 @@ def load(ncol):
         "susy_sg_loss_grad": (i, [vp, dp, dp, dp, dp]), "susy_sg_record": (i, [vp, i]),
         "susy_sg_pairs": (i, [vp]), "susy_sg_pair_loss": (i, [vp, i, dp]),
         "susy_sg_train": (i, [vp, i, i, i, i, d, i, dp]),
+        "susy_eig_dsq": (i, [vp, i, i, i, d, i, dp, dp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -282,6 +283,14 @@ class Lattice:
         self._check(self.L.susy_prof_read(self.h, prec, ctypes.byref(ms), ctypes.byref(n),
                                           ctypes.byref(k)))
         return ms.value, n.value, k.value
+
+    def eig_dsq(self, nlow, nhigh, maxiter=600, tol=1e-10, seed=1):
+        """(nlow smallest, nhigh largest eigenvalues of DSq, Lanczos iterations)"""
+        lo, hi = np.zeros(max(nlow, 1)), np.zeros(max(nhigh, 1))
+        it = ctypes.c_int()
+        self._check(self.L.susy_eig_dsq(self.h, nlow, nhigh, maxiter, tol, seed, _dp(lo),
+                                        _dp(hi), ctypes.byref(it)))
+        return lo[:nlow], hi[:nhigh], it.value
 
     # ---- learned surrogate
     def sg_init(self, seed=1, scale=1.0):
