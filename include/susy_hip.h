@@ -213,6 +213,20 @@ int susy_set_site_order(susy_ctx *c, int paired);
    inner (gauge) force and removes it from every outer (fermion) kick, so the
    outer level integrates S_F - S_theta and the exact action still decides
    the Metropolis test. */
+/* MILC serial gauge file of the context's links, readable by
+   susy_load_config and by the reference's reload_serial (io_lat.c); stamp: the
+   64-byte time stamp to record, or NULL for the current time. */
+int susy_save_config(susy_ctx *c, const char *path, const char *stamp);
+/* Measurements the reference prints after every trajectory (control.c:85-120):
+   FLINK (link_trace.c) -- Tr(U_a U_a^dag)/N per direction, average, width, and
+   Re det of the traceless part of U_a U_a^dag per direction, average, width;
+   Polyakov / Wilson lines (ploop.c) in link direction a = 0..4 (GMES uses
+   a = 3, LINES all five): average trace over the starting hyperplane and the
+   average modulus. */
+int susy_link_trace_dev(susy_ctx *c, double *linktr, double *ave, double *width,
+                        double *link_det, double *det_ave, double *det_width);
+int susy_wilson_line_dev(susy_ctx *c, int a, double *re, double *im, double *mod);
+
 /* Extreme eigenvalues of DSq (D^dag D + fmass^2) at the context's links, the
    measurement the reference prints in testsuite/scalar/eig.U*.ref (there
    with PRIMME): Lanczos with full reorthogonalisation on the FP64 operator,

@@ -49,7 +49,8 @@ EXPORTS = [
     "susy_set_verbose", "susy_stream_copy_dev", "susy_set_site_order",
     "susy_sg_nparams", "susy_sg_init", "susy_sg_set_params", "susy_sg_get_params",
     "susy_sg_eval", "susy_sg_features", "susy_sg_loss_grad", "susy_sg_record", "susy_sg_pairs",
-    "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq",
+    "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
+    "susy_link_trace_dev", "susy_wilson_line_dev",
 ]
 
 _libs = {}
@@ -108,6 +109,9 @@ def load(ncol):
         "susy_sg_pairs": (i, [vp]), "susy_sg_pair_loss": (i, [vp, i, dp]),
         "susy_sg_train": (i, [vp, i, i, i, i, d, i, dp]),
         "susy_eig_dsq": (i, [vp, i, i, i, d, i, dp, dp, ip]),
+        "susy_save_config": (i, [vp, ctypes.c_char_p, ctypes.c_char_p]),
+        "susy_link_trace_dev": (i, [vp, dp, dp, dp, dp, dp, dp]),
+        "susy_wilson_line_dev": (i, [vp, i, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -283,6 +287,27 @@ class Lattice:
         self._check(self.L.susy_prof_read(self.h, prec, ctypes.byref(ms), ctypes.byref(n),
                                           ctypes.byref(k)))
         return ms.value, n.value, k.value
+
+    def save_config(self, path, stamp=None):
+        """MILC serial gauge file; stamp: 64 bytes or None (current time)"""
+        st = None if stamp is None else bytes(stamp).ljust(64, b"\0")[:64]
+        self._check(self.L.susy_save_config(self.h, os.fsencode(path), st))
+
+    def link_trace(self):
+        """(Tr(UU^dag)/N per direction, average, width,
+            det of the traceless part per direction, average, width)"""
+        lt, ld = np.zeros(5), np.zeros(5)
+        a, w, da, dw = (ctypes.c_double() for _ in range(4))
+        self._check(self.L.susy_link_trace_dev(self.h, _dp(lt), ctypes.byref(a), ctypes.byref(w),
+                                               _dp(ld), ctypes.byref(da), ctypes.byref(dw)))
+        return lt, a.value, w.value, ld, da.value, dw.value
+
+    def wilson_line(self, a):
+        """(Re, Im, average modulus) of the Wilson line in link direction a"""
+        re, im, mod = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.susy_wilson_line_dev(self.h, a, ctypes.byref(re), ctypes.byref(im),
+                                                ctypes.byref(mod)))
+        return re.value, im.value, mod.value
 
     def eig_dsq(self, nlow, nhigh, maxiter=600, tol=1e-10, seed=1):
         """(nlow smallest, nhigh largest eigenvalues of DSq, Lanczos iterations)"""

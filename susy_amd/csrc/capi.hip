@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <chrono>
@@ -363,6 +364,62 @@ extern "C" int susy_load_config(susy_ctx *c, const char *path) {
   fclose(fp);
   RET(susy_set_links(c, links.data()));
   return (c29 == sums[0] && c31 == sums[1]) ? 0 : 1;
+}
+
+// MILC serial gauge file of the context's links (the format susy_load_config
+// reads; io_lat.c): header (magic 20103, nx ny nz nt, 64-byte time stamp,
+// site order 0, the two rotating-XOR checksums of the payload), then per
+// site in lexicographic order the 5 links as single-precision complex.
+// stamp: 64 bytes to record, or NULL for the current time.
+extern "C" int susy_save_config(susy_ctx *c, const char *path, const char *stamp) {
+  if (!c || !path) return SUSY_E_ARG;
+  const int V = c->V;
+  const size_t nw = (size_t)NUMLINK * NN * 2;
+  std::vector<double> links((size_t)V * nw);
+  RET(susy_get_links(c, links.data()));
+  std::vector<float> pay((size_t)V * nw);
+  uint32_t c29 = 0, c31 = 0;
+  int r29 = 0, r31 = 0;
+  for (int lex = 0; lex < V; lex++) {
+    const int x = lex % c->p.nx, y = (lex / c->p.nx) % c->p.ny;
+    const int z = (lex / (c->p.nx * c->p.ny)) % c->p.nz, t = lex / (c->p.nx * c->p.ny * c->p.nz);
+    const int i = site_index(c->geo, x, y, z, t);
+    for (size_t k = 0; k < nw; k++) {
+      const float f = (float)links[(size_t)i * nw + k];
+      pay[(size_t)lex * nw + k] = f;
+      uint32_t v;
+      memcpy(&v, &f, 4);
+      c29 ^= r29 ? (v << r29 | v >> (32 - r29)) : v;
+      c31 ^= r31 ? (v << r31 | v >> (32 - r31)) : v;
+      if (++r29 >= 29) r29 = 0;
+      if (++r31 >= 31) r31 = 0;
+    }
+  }
+  uint32_t hdr[5] = {20103u, (uint32_t)c->p.nx, (uint32_t)c->p.ny, (uint32_t)c->p.nz,
+                     (uint32_t)c->p.nt};
+  char st[64];
+  memset(st, 0, sizeof(st));
+  if (stamp) {
+    memcpy(st, stamp, 64);
+  } else {
+    time_t now = time(nullptr);
+    strftime(st, sizeof(st), "%a %b %e %H:%M:%S %Y", localtime(&now));
+  }
+  const uint32_t order = 0, sums[2] = {c29, c31};
+  FILE *fp = fopen(path, "wb");
+  if (!fp) {
+    set_error("cannot create %s", path);
+    return SUSY_E_ARG;
+  }
+  bool ok = fwrite(hdr, 4, 5, fp) == 5 && fwrite(st, 1, 64, fp) == 64 &&
+            fwrite(&order, 4, 1, fp) == 1 && fwrite(sums, 4, 2, fp) == 2 &&
+            fwrite(pay.data(), sizeof(float), pay.size(), fp) == pay.size();
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) {
+    set_error("%s: write failed", path);
+    return SUSY_E_ARG;
+  }
+  return 0;
 }
 
 // ---------------------------------------------------------------------
