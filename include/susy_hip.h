@@ -200,32 +200,43 @@ int susy_time_fermion_op(susy_ctx *c, int h_src, int h_dst, int form,
 // susy_prof_read returns the average sampled duration in ms and the number
 // of D launches for prec 0 (FP64 kernel) or 1 (FP32 kernel).
 int susy_prof_enable(susy_ctx *c, int every);
+int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
+                   long *sampled);
 /* Block order of the operator kernel (a tuning knob; results do not depend
-   on it): 0 = each XCD walks a contiguous range of sites, 1 (default) = each
-   XCD walks the same range of the even and the odd half of the site order. */
+   on it): 0 = each XCD walks a contiguous range of sites, 1 = each XCD walks
+   the same range of the even and the odd half of the site order, 2 = as 1
+   with the range visited t-slice fastest (default; a lattice whose slabs
+   are not whole t-slices of whole chunks runs order 1).  A value below 256 sets both
+   precisions; otherwise the low byte is the FP64 order and the next byte the
+   FP32 order. */
 int susy_set_site_order(susy_ctx *c, int paired);
 
-/* Learned surrogate action S_theta(U) (no counterpart in the reference; its
-   judge is the plain fermion_force).  Two gauge-invariant convolution layers
-   (16 channels, bfloat16 weights, FP32 accumulation) over plaquette and
-   1x2-loop traces; F_theta = dS_theta/dUbar in the convention of
-   fermion_force.  With SUSY_FORM_SURROGATE, susy_update adds F_theta to every
-   inner (gauge) force and removes it from every outer (fermion) kick, so the
-   outer level integrates S_F - S_theta and the exact action still decides
-   the Metropolis test. */
 /* MILC serial gauge file of the context's links, readable by
    susy_load_config and by the reference's reload_serial (io_lat.c); stamp: the
    64-byte time stamp to record, or NULL for the current time. */
 int susy_save_config(susy_ctx *c, const char *path, const char *stamp);
-/* Measurements the reference prints after every trajectory (control.c:85-120):
+/* Measurements the reference prints after every trajectory (control.c:85-138):
    FLINK (link_trace.c) -- Tr(U_a U_a^dag)/N per direction, average, width, and
    Re det of the traceless part of U_a U_a^dag per direction, average, width;
-   Polyakov / Wilson lines (ploop.c) in link direction a = 0..4 (GMES uses
-   a = 3, LINES all five): average trace over the starting hyperplane and the
-   average modulus. */
+   Polyakov / Wilson lines (ploop.c, ploop(dir, project, &mod)) in link
+   direction a = 0..4 (GMES uses a = 3, LINES all five, LINES_POLAR all five
+   with project = 1: links replaced by the unitary factor of their polar
+   decomposition): average trace over the starting hyperplane and the
+   average modulus;
+   UUBAR_EIG / POLAR_EIG (scalar_eig.c, scalar_eig(project, ...)) -- per
+   eigenvalue index (ascending) the average, width, minimum and maximum over
+   all links of the eigenvalues of the traceless part of U U^dag (project 0)
+   or of log (U U^dag)^(1/2) (project 1); arrays of NCOL;
+   DET (determinant.c:93-119, measure_det) -- det5 = Re, Im <plaqdet>,
+   <Re^2>, <Im^2>, <|plaqdet - 1|^2>, and WIDTHS (widths.c) -- w3 = widths
+   of Re Tr, Re det, Im det of the plaquettes. */
 int susy_link_trace_dev(susy_ctx *c, double *linktr, double *ave, double *width,
                         double *link_det, double *det_ave, double *det_width);
-int susy_wilson_line_dev(susy_ctx *c, int a, double *re, double *im, double *mod);
+int susy_wilson_line_dev(susy_ctx *c, int a, int project, double *re, double *im,
+                         double *mod);
+int susy_scalar_eig_dev(susy_ctx *c, int project, double *ave, double *width, double *mn,
+                        double *mx);
+int susy_plaq_moments_dev(susy_ctx *c, double *det5, double *w3);
 
 /* Extreme eigenvalues of DSq (D^dag D + fmass^2) at the context's links, the
    measurement the reference prints in testsuite/scalar/eig.U*.ref (there
@@ -237,6 +248,14 @@ int susy_wilson_line_dev(susy_ctx *c, int a, double *re, double *im, double *mod
 int susy_eig_dsq(susy_ctx *c, int nlow, int nhigh, int maxiter, double tol, int seed,
                  double *low, double *high, int *iters);
 
+/* Learned surrogate action S_theta(U) (no counterpart in the reference; its
+   judge is the plain fermion_force).  Two gauge-invariant convolution layers
+   (16 channels, bfloat16 weights, FP32 accumulation) over plaquette and
+   1x2-loop traces; F_theta = dS_theta/dUbar in the convention of
+   fermion_force.  With SUSY_FORM_SURROGATE, susy_update adds F_theta to every
+   inner (gauge) force and removes it from every outer (fermion) kick, so the
+   outer level integrates S_F - S_theta and the exact action still decides
+   the Metropolis test. */
 int susy_sg_nparams(void);
 int susy_sg_init(susy_ctx *c, int seed, double scale);
 int susy_sg_set_params(susy_ctx *c, const float *theta);
@@ -256,8 +275,6 @@ int susy_sg_pair_loss(susy_ctx *c, int i, double *loss);
 /* Adam over recorded pairs [first, first + count) */
 int susy_sg_train(susy_ctx *c, int first, int count, int steps, int batch, double lr, int seed,
                   double *loss_hist);
-int susy_prof_read(susy_ctx *c, int prec, double *avg_ms, long *launches,
-                   long *sampled);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,9 @@ LIBDIR = os.path.join(HERE, "lib")
 FORM_FP64 = 0
 FORM_FP32 = 1
 FORM_SURROGATE = 4      # flag: learned surrogate force on the inner MD level
+# operator-kernel block order a new context starts with (susy_set_site_order:
+# FP64 order in the low byte, FP32 order in the next)
+DEFAULT_SITE_ORDER = 2 + (2 << 8)
 NGATHER = 74
 
 
@@ -50,7 +53,8 @@ EXPORTS = [
     "susy_sg_nparams", "susy_sg_init", "susy_sg_set_params", "susy_sg_get_params",
     "susy_sg_eval", "susy_sg_features", "susy_sg_loss_grad", "susy_sg_record", "susy_sg_pairs",
     "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
-    "susy_link_trace_dev", "susy_wilson_line_dev",
+    "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
+    "susy_plaq_moments_dev",
 ]
 
 _libs = {}
@@ -111,7 +115,9 @@ def load(ncol):
         "susy_eig_dsq": (i, [vp, i, i, i, d, i, dp, dp, ip]),
         "susy_save_config": (i, [vp, ctypes.c_char_p, ctypes.c_char_p]),
         "susy_link_trace_dev": (i, [vp, dp, dp, dp, dp, dp, dp]),
-        "susy_wilson_line_dev": (i, [vp, i, dp, dp, dp]),
+        "susy_wilson_line_dev": (i, [vp, i, i, dp, dp, dp]),
+        "susy_scalar_eig_dev": (i, [vp, i, dp, dp, dp, dp]),
+        "susy_plaq_moments_dev": (i, [vp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -302,12 +308,27 @@ class Lattice:
                                                _dp(ld), ctypes.byref(da), ctypes.byref(dw)))
         return lt, a.value, w.value, ld, da.value, dw.value
 
-    def wilson_line(self, a):
-        """(Re, Im, average modulus) of the Wilson line in link direction a"""
+    def wilson_line(self, a, project=False):
+        """(Re, Im, average modulus) of the Wilson line in link direction a
+        (project: of the polar-projected unitary links)"""
         re, im, mod = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        self._check(self.L.susy_wilson_line_dev(self.h, a, ctypes.byref(re), ctypes.byref(im),
-                                                ctypes.byref(mod)))
+        self._check(self.L.susy_wilson_line_dev(self.h, a, int(project), ctypes.byref(re),
+                                                ctypes.byref(im), ctypes.byref(mod)))
         return re.value, im.value, mod.value
+
+    def scalar_eig(self, project=False):
+        """(average, width, minimum, maximum) per ascending eigenvalue index of
+        the traceless part of U U^dag, or of log (U U^dag)^(1/2) (project)"""
+        out = [np.zeros(self.ncol) for _ in range(4)]
+        self._check(self.L.susy_scalar_eig_dev(self.h, int(project), *(_dp(o) for o in out)))
+        return tuple(out)
+
+    def plaq_moments(self):
+        """(DET: Re, Im <plaqdet>, <Re^2>, <Im^2>, <|plaqdet - 1|^2>;
+            WIDTHS: widths of Re Tr P, Re det P, Im det P)"""
+        d5, w3 = np.zeros(5), np.zeros(3)
+        self._check(self.L.susy_plaq_moments_dev(self.h, _dp(d5), _dp(w3)))
+        return d5, w3
 
     def eig_dsq(self, nlow, nhigh, maxiter=600, tol=1e-10, seed=1):
         """(nlow smallest, nhigh largest eigenvalues of DSq, Lanczos iterations)"""

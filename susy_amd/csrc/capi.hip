@@ -174,7 +174,9 @@ static int ctx_setup(susy_ctx *c) {
   RET(dalloc(&c->f_aux, (size_t)6 * V));
   RET(dalloc(&c->d_tmp, tfn));
   RET(dalloc(&c->d_boundary, tfn));
-  int ds_blocks = std::max(ds_grid(V, 0, nullptr), ds_grid(V, 1, nullptr));
+  const int tsl = p.nx * p.ny * p.nz;
+  int ds_blocks = std::max({ds_grid(V, tsl, 0, nullptr), ds_grid(V, tsl, 1, nullptr),
+                            ds_grid(V, tsl, 2, nullptr)});
   c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
   RET(dalloc(&c->d_part, (size_t)c->npart * 2));
   RET(dalloc(&c->d_scal, (size_t)64));
@@ -773,11 +775,15 @@ static int traj_force(Traj &T, int root, double eps) {
 
 static int g_verbose = 0;
 extern "C" int susy_set_site_order(susy_ctx *c, int paired) {
-  if (!c || (paired != 0 && paired != 1)) {
-    set_error("susy_set_site_order: paired must be 0 or 1");
+  // paired: FP64 order in the low byte, FP32 order in the next (a value
+  // below 256 sets both)
+  const int o64 = paired & 255, o32 = paired >= 256 ? (paired >> 8) & 255 : o64;
+  if (!c || paired < 0 || o64 > 2 || o32 > 2) {
+    set_error("susy_set_site_order: orders must be 0, 1 or 2");
     return SUSY_E_ARG;
   }
-  c->site_order = paired;
+  c->site_order = o64;
+  c->site_order32 = o32;
   return 0;
 }
 

@@ -155,32 +155,55 @@ __global__ void k_cgs_beta(CgState *st, const double *__restrict__ cd) {
 }
 
 // alpha, the window coefficients, convergence of the shifted systems and the
-// advance of the recurrences after |r_new|^2 = *rsqnew
-__global__ void k_cgs_alpha(CgState *st, const double *__restrict__ rsqnew) {
+// advance of the recurrences after |r_new|^2 = *rsqnew.  One block of
+// MAXSH x MAXWIN threads: thread (j, l) updates the window coefficients
+// Cw[j][l], Dw[j][l]; thread l = 0 of shift j does that shift's scalars.
+// Every element sees the same operations in the same order as a serial
+// loop over j and l would make.
+#define ALPHA_THREADS (MAXSH * MAXWIN)
+__global__ void __launch_bounds__(ALPHA_THREADS)
+k_cgs_alpha(CgState *st, const double *__restrict__ rsqnew) {
 #pragma clang fp contract(off)
   if (st->done) return;
+  __shared__ double al_s[MAXSH], bi_s[MAXSH];
+  __shared__ int conv_s[MAXSH];
+  const int j = threadIdx.x / MAXWIN, l = threadIdx.x % MAXWIN;
+  const int nsh = st->nsh, cnt = st->cnt;
+  const double rn = *rsqnew, rsq = st->rsq;
   double *zi = st->zi, *zim1 = st->zim1, *zip1 = st->zip1, *bi = st->bi, *bim1 = st->bim1,
          *al = st->al;
-  const int nsh = st->nsh, cnt = st->cnt;
-  al[0] = *rsqnew / st->rsq;
-  for (int j = 1; j < nsh; j++)
-    if (!st->conv[j]) al[j] = al[0] * zip1[j] * bi[j] / (zi[j] * bi[0]);
-  // psi_j -= beta_j p_j, then p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
-  for (int j = 0; j < nsh; j++) {
-    if (st->conv[j]) continue;
-    st->touched[j] = 1;
-    st->E[j] -= bi[j] * st->A[j];
-    for (int l = 0; l < cnt; l++) st->Dw[j][l] -= bi[j] * st->Cw[j][l];
-    st->A[j] *= al[j];
-    for (int l = 0; l < cnt; l++) st->Cw[j][l] *= al[j];
-    st->Cw[j][cnt] = zip1[j];
+  if (l == 0 && j < nsh) {
+    const int cv = st->conv[j];
+    const double a0 = rn / rsq;
+    if (j == 0) al[0] = a0;
+    else if (!cv) al[j] = a0 * zip1[j] * bi[j] / (zi[j] * bi[0]);
+    al_s[j] = al[j];
+    bi_s[j] = bi[j];
+    conv_s[j] = cv;
   }
-  st->cnt = cnt + 1;
-  st->rsq = *rsqnew;
-  for (int j = 1; j < nsh; j++)
-    if (!st->conv[j] && st->rsq * zip1[j] * zip1[j] <= st->rsqstop) st->conv[j] = 1;
-  for (int j = 0; j < nsh; j++)
-    if (!st->conv[j]) { bim1[j] = bi[j]; zim1[j] = zi[j]; zi[j] = zip1[j]; }
+  __syncthreads();   // every thread has read cnt, rsq and the old conv
+  if (j < nsh && !conv_s[j]) {
+    // psi_j -= beta_j p_j, then p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
+    if (l < cnt) {
+      st->Dw[j][l] -= bi_s[j] * st->Cw[j][l];
+      st->Cw[j][l] *= al_s[j];
+    }
+    if (l == 0) {
+      st->touched[j] = 1;
+      st->E[j] -= bi_s[j] * st->A[j];
+      st->A[j] *= al_s[j];
+      st->Cw[j][cnt] = zip1[j];
+    }
+  }
+  if (l == 0 && j < nsh) {
+    int cv = conv_s[j];
+    if (j > 0 && !cv && rn * zip1[j] * zip1[j] <= st->rsqstop) cv = st->conv[j] = 1;
+    if (!cv) { bim1[j] = bi_s[j]; zim1[j] = zi[j]; zi[j] = zip1[j]; }
+  }
+  if (threadIdx.x == 0) {
+    st->cnt = cnt + 1;
+    st->rsq = rn;
+  }
 }
 
 // window state after a flush
@@ -468,7 +491,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
                            c->d_part);
         HIPCHK(hipGetLastError());
         if ((e = reduce_partials(c, c->d_part, nb, 2))) return e;
-        hipLaunchKernelGGL(k_cgs_alpha, dim3(1), dim3(1), 0, c->s, st, d_rn);
+        hipLaunchKernelGGL(k_cgs_alpha, dim3(1), dim3(ALPHA_THREADS), 0, c->s, st, d_rn);
         C *pnext = pcur != pfrozen ? pcur : (pcur == p0buf[0] ? p0buf[1] : p0buf[0]);
         hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, pcur,
                            rnew, pnext, st, n);

@@ -202,7 +202,8 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   bool live;
   if (ord.paired) {
     // even chunk k, odd chunk k, even chunk k+1, ... of this XCD's slab
-    const int par = qq & 1, chunk = (L & 7) * ord.slab + (qq >> 1);
+    const int par = qq & 1, pos = qq >> 1;
+    const int chunk = (L & 7) * ord.slab + (ord.paired == 2 ? (pos % ord.T) * ord.P + pos / ord.T : pos);
     const int i = chunk * DS_SITES + lane;
     s = par ? ord.half + i : i;
     live = i < (par ? V - ord.half : ord.half);
@@ -442,7 +443,8 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     HIPCHK(hipGetLastError());
   }
   DsOrder ord;
-  const int nb = ds_grid(V, c->site_order, &ord);
+  const int nb = ds_grid(V, c->p.nx * c->p.ny * c->p.nz,
+                         sizeof(R) == sizeof(float) ? c->site_order32 : c->site_order, &ord);
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \

@@ -41,18 +41,34 @@
 //   paired = 1: a slab is the same chunk range in the even half [0, half) and
 //     the odd half [half, V) of the even-first site order, even and odd chunks
 //     alternating, so a site's nearest neighbours (opposite parity, same t
-//     range) are fetched into the same L2.
+//     range) are fetched into the same L2;
+//   paired = 2: as 1, with the slab's chunks visited t fastest (P chunks per
+//     t-slice and parity, T t-slices per slab: chunk (pos % T) P + pos / T),
+//     so t-neighbours are a chunk apart instead of a t-slice.  Needs the
+//     slab to hold whole t-slices of whole chunks; otherwise it is order 1.
 struct DsOrder {
-  int paired, slab, half;
+  int paired, slab, half, P, T;
 };
-// grid size (blocks) of the operator kernel, and its order parameters
-inline int ds_grid(int V, int paired, DsOrder *o) {
+// grid size (blocks) of the operator kernel, and its order parameters;
+// tslice = sites per t-slice (nx ny nz)
+inline int ds_grid(int V, int tslice, int paired, DsOrder *o) {
   DsOrder d;
   d.paired = paired;
   d.half = V / 2;
+  d.P = d.T = 1;
   if (paired) {
     const int chunks = (V - d.half + DS_SITES - 1) / DS_SITES;   // odd half is the larger
     d.slab = (chunks + 7) / 8;
+    if (paired == 2) {
+      const bool fit = V % 2 == 0 && tslice % (2 * DS_SITES) == 0 && d.half % DS_SITES == 0 &&
+                       chunks % 8 == 0 && d.slab % (tslice / (2 * DS_SITES)) == 0;
+      if (fit) {
+        d.P = tslice / (2 * DS_SITES);
+        d.T = d.slab / d.P;
+      } else {
+        d.paired = 1;
+      }
+    }
     if (o) *o = d;
     return 8 * d.slab * 2 * NTF;
   }
@@ -199,7 +215,8 @@ struct susy_ctx {
   int total_iters = 0;
   // sampled HIP-event timing of the operator kernel launches (per precision:
   // index 0 FP64, 1 FP32): every prof_every-th launch is bracketed by events
-  int site_order = 1;          // DsOrder::paired of the operator kernel
+  int site_order = 2;          // DsOrder::paired of the operator kernel (FP64)
+  int site_order32 = 2;        // the same for the FP32 operator
   SgState *sg = nullptr;       // learned surrogate state (susy_sg_init)
   int prof_every = 0;
   long prof_launch[2] = {0, 0};

@@ -210,11 +210,17 @@ def test_larger_volume(ncol):
         lat.fermion_op(h, h + 1, sign, form=susy_amd.FORM_FP32)
         got32 = lat.tf_download(h + 1)
         assert rel_err(got32, want) < 1e-5
-        # the block order is a schedule only: the other order is bit-identical
-        lat.set_site_order(0)
-        lat.fermion_op(h, h + 1, sign, form=susy_amd.FORM_FP32)
-        assert np.array_equal(lat.tf_download(h + 1), got32)
-        lat.set_site_order(1)
+        # the block order is a schedule only: every order is bit-identical
+        # (8^3 x 16 admits order 2: 2 chunks per t-slice and parity)
+        lat.fermion_op(h, h + 1, sign)
+        got64 = lat.tf_download(h + 1)
+        for order in (0, 1, 2, 1 + (2 << 8), 2 + (0 << 8) + 256):
+            lat.set_site_order(order)
+            lat.fermion_op(h, h + 1, sign, form=susy_amd.FORM_FP32)
+            assert np.array_equal(lat.tf_download(h + 1), got32), order
+            lat.fermion_op(h, h + 1, sign)
+            assert np.array_equal(lat.tf_download(h + 1), got64), order
+        lat.set_site_order(susy_amd.DEFAULT_SITE_ORDER)
     mom = random_tf(O.link_shape(), 43)
     O.set_mom(mom)
     lat.set_mom(mom)
