@@ -96,6 +96,7 @@ static void free_ctx(susy_ctx *c) {
   void *ptrs[] = {c->d_nbr, c->d_t, c->d_links, c->d_mom, c->d_old_links, c->d_old_mom,
                   c->d_uinv, c->d_det, c->d_pdet, c->d_zw, c->d_dmu, c->d_fmunu,
                   c->d_force, c->f_links, c->f_uinv, c->f_pdet, c->d_aux, c->f_aux,
+                  c->d_aux2, c->f_aux2,
                   c->d_tmp, c->f_tmp, c->d_boundary, c->d_part, c->d_scal, c->cg_ws,
                   c->cg_t64, c->cg_state, c->f_in, c->f_out};
   for (void *p : ptrs)
@@ -172,6 +173,8 @@ static int ctx_setup(susy_ctx *c) {
   RET(dalloc(&c->f_pdet, (size_t)NUMLINK * NUMLINK * V));
   RET(dalloc(&c->d_aux, (size_t)12 * V));
   RET(dalloc(&c->f_aux, (size_t)6 * V));
+  RET(dalloc(&c->d_aux2, (size_t)6 * V));
+  RET(dalloc(&c->f_aux2, (size_t)6 * V));
   RET(dalloc(&c->d_tmp, tfn));
   RET(dalloc(&c->d_boundary, tfn));
   const int tsl = p.nx * p.ny * p.nz;
@@ -492,9 +495,8 @@ extern "C" int susy_fermion_op_dev(susy_ctx *c, int hs, int hd, int sign, int fo
 // dest = D^dag D src + fmass^2 src (utilities.c:874-885)
 static int dsq64(susy_ctx *c, const double2 *s, double2 *d) {
   double f2 = c->p.fmass > IMAG_TOL ? c->p.fmass * c->p.fmass : 0.0;
-  RET(dslash64(c, s, c->d_tmp, 1, nullptr, 0.0, nullptr, nullptr));
   int nb;
-  return dslash64(c, c->d_tmp, d, -1, s, f2, c->d_part, &nb);
+  return dsq_pair64(c, s, c->d_tmp, d, f2, c->d_part, &nb);
 }
 extern "C" int susy_stream_copy_dev(susy_ctx *c, int hs, int hd) {
   double2 *s = tfh(c, hs), *d = tfh(c, hd);
@@ -512,9 +514,8 @@ extern "C" int susy_dsq_dev(susy_ctx *c, int hs, int hd, int form) {
     size_t n = tf_elems(c);
     float f2 = c->p.fmass > IMAG_TOL ? (float)(c->p.fmass * c->p.fmass) : 0.f;
     hipLaunchKernelGGL(k_to32, dim3((n + 255) / 256), dim3(256), 0, c->s, s, c->f_in, n);
-    RET(dslash32(c, c->f_in, c->f_tmp, 1, nullptr, 0.f, nullptr, nullptr));
     int nb;
-    RET(dslash32(c, c->f_tmp, c->f_out, -1, c->f_in, f2, c->d_part, &nb));
+    RET(dsq_pair32(c, c->f_in, c->f_tmp, c->f_out, f2, c->d_part, &nb));
     hipLaunchKernelGGL(k_to64, dim3((n + 255) / 256), dim3(256), 0, c->s, c->f_out, d, n);
   } else {
     RET(dsq64(c, s, d));

@@ -431,15 +431,11 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   auto dsq = [&](const C *in, C *out, int *nbo) -> int {
     int r1, nbd = 0;
     if (fp32) {
-      r1 = dslash32(c, (const float2 *)in, (float2 *)tmp, 1, nullptr, 0.f, nullptr, nullptr,
-                    d_done);
-      if (!r1) r1 = dslash32(c, (const float2 *)tmp, (float2 *)out, -1, (const float2 *)in,
-                             (float)sigma0, c->d_part, &nbd, d_done);
+      r1 = dsq_pair32(c, (const float2 *)in, (float2 *)tmp, (float2 *)out, (float)sigma0,
+                      c->d_part, &nbd, d_done);
     } else {
-      r1 = dslash64(c, (const double2 *)in, (double2 *)tmp, 1, nullptr, 0.0, nullptr, nullptr,
-                    d_done);
-      if (!r1) r1 = dslash64(c, (const double2 *)tmp, (double2 *)out, -1, (const double2 *)in,
-                             sigma0, c->d_part, &nbd, d_done);
+      r1 = dsq_pair64(c, (const double2 *)in, (double2 *)tmp, (double2 *)out, sigma0,
+                      c->d_part, &nbd, d_done);
     }
     *nbo = nbd;
     return r1;
@@ -449,8 +445,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   auto reliable_update = [&](const int *skip) -> int {
     int r1, nbd = 0;
     if ((r1 = flush(skip))) return r1;
-    r1 = dslash64(c, psim[0], tmp64, 1, nullptr, 0.0, nullptr, nullptr, skip);
-    if (!r1) r1 = dslash64(c, tmp64, ap64, -1, psim[0], sigma0, c->d_part, &nbd, skip);
+    r1 = dsq_pair64(c, psim[0], tmp64, ap64, sigma0, c->d_part, &nbd, skip);
     if (r1) return r1;
     hipLaunchKernelGGL((k_true_res<C>), dim3(nb), dim3(CB), 0, c->s, src, ap64, r, n, c->d_part,
                        skip);

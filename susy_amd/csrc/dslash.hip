@@ -173,6 +173,9 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
 
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
+// With aux_out (D, EPI = 0 only) the blocks of the site and link components
+// also store the pre-pass values of D^dagger applied to this output:
+// aux_out[0] = Tr(out_0)^*, aux_out[1+a] = Tr(U_a^-1 out_{1+a}^dag).
 //
 // A block covers DS_SITES sites of one output component.  With DS_SPLIT = 2
 // two threads share each output matrix: the first computes the site terms
@@ -185,6 +188,7 @@ __global__ void __launch_bounds__(DS_THREADS) __attribute__((amdgpu_waves_per_eu
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
+         typename CT<R>::t *__restrict__ aux_out,
          const int *__restrict__ nbr, const int *__restrict__ tco, int V, int nt,
          R pbc, int doG, R G, DsOrder ord,
          const typename CT<R>::t *__restrict__ p0,
@@ -401,6 +405,28 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
         }
         bstore<C>(v, rD, (uint32_t)s * cs, so);
       }
+    if (!DAG && EPI == 0 && aux_out != nullptr && comp <= NUMLINK) {
+      C tr = mkc<C>(0, 0);
+      if (comp == 0) {
+#pragma unroll
+        for (int i = 0; i < NCOL; i++) {
+          tr.x += acc[i][i].x;
+          tr.y -= acc[i][i].y;
+        }
+      } else {
+        const Rsrc rI = make_rsrc(Uinv, NUMLINK * Mb);
+#pragma unroll
+        for (int i = 0; i < NCOL; i++)
+#pragma unroll
+          for (int j = 0; j < NCOL; j++) {
+            const C u = bload<C>(rI, (uint32_t)s * cs, (comp - 1) * Mb + (i * NCOL + j) * pl);
+            const C p = acc[i][j];
+            tr.x += u.x * p.x + u.y * p.y;
+            tr.y += u.y * p.x - u.x * p.y;
+          }
+      }
+      aux_out[(size_t)comp * V + s] = tr;
+    }
   }
   if (EPI == 1) {
     // deterministic block reduction: wavefront butterfly, then across waves
@@ -422,7 +448,8 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
                          R shift, double *partial, int *nblocks,
                          const typename CT<R>::t *U, const typename CT<R>::t *Uinv,
                          const typename CT<R>::t *pdet, typename CT<R>::t *aux,
-                         const int *skip) {
+                         const int *skip, typename CT<R>::t *aux_out = nullptr,
+                         bool prep = true) {
   if (sign != 1 && sign != -1) {
     set_error("fermion_op: incorrect sign %d", sign);
     return SUSY_E_ARG;
@@ -432,7 +459,8 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     set_error("lattice too large for 32-bit buffer offsets (V=%d)", V);
     return SUSY_E_ARG;
   }
-  if (c->doG) {
+  if (sign != 1 || p0 != nullptr || !c->doG) aux_out = nullptr;
+  if (c->doG && prep) {
     int nb = (V + 255) / 256;
     if (sign == 1)
       hipLaunchKernelGGL((k_dslash_prep<R, false>), dim3(nb), dim3(256), 0, c->s, src, Uinv, aux, V,
@@ -447,8 +475,8 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
                          sizeof(R) == sizeof(float) ? c->site_order32 : c->site_order, &ord);
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
-#define DS_ARGS src, dst, U, Uinv, pdet, aux, c->d_nbr, c->d_t, V, c->p.nt, pbc, c->doG, G, \
-                ord, p0, shift, partial, skip
+#define DS_ARGS src, dst, U, Uinv, pdet, aux, aux_out, c->d_nbr, c->d_t, V, c->p.nt, pbc, \
+                c->doG, G, ord, p0, shift, partial, skip
   // sampled timing covers the plain operator D (sign +1, no epilogue) only
   const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -526,4 +554,29 @@ int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
   }
   return dslash_launch<float>(c, src, dst, sign, p0, shift, partial, nblocks,
                               c->f_links, c->f_uinv, c->f_pdet, c->f_aux, skip);
+}
+
+// dst = D^dagger D src + shift src, tmp = D src, per-block partials of
+// Re <src, dst> in partial (the DSq of utilities.c:874-885 with the CG's
+// shift).  D stores the D^dagger pre-pass values of its output (aux2), so
+// D^dagger runs without its pre-pass.
+int dsq_pair64(susy_ctx *c, const double2 *src, double2 *tmp, double2 *dst, double shift,
+               double *partial, int *nblocks, const int *skip) {
+  int r = dslash_launch<double>(c, src, tmp, 1, nullptr, 0.0, nullptr, nullptr, c->d_links,
+                                c->d_uinv, c->d_pdet, c->d_aux, skip, c->d_aux2);
+  if (r) return r;
+  return dslash_launch<double>(c, tmp, dst, -1, src, shift, partial, nblocks, c->d_links,
+                               c->d_uinv, c->d_pdet, c->d_aux2, skip, nullptr, false);
+}
+int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float shift,
+               double *partial, int *nblocks, const int *skip) {
+  if (!c->f_fresh) {
+    int r = fields_refresh32(c);
+    if (r) return r;
+  }
+  int r = dslash_launch<float>(c, src, tmp, 1, nullptr, 0.f, nullptr, nullptr, c->f_links,
+                               c->f_uinv, c->f_pdet, c->f_aux, skip, c->f_aux2);
+  if (r) return r;
+  return dslash_launch<float>(c, tmp, dst, -1, src, shift, partial, nblocks, c->f_links,
+                              c->f_uinv, c->f_pdet, c->f_aux2, skip, nullptr, false);
 }

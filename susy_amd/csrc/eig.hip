@@ -159,8 +159,7 @@ extern "C" int susy_eig_dsq(susy_ctx *c, int nlow, int nhigh, int maxiter, doubl
     double2 *vj = V + (size_t)j * n;
     int nbd = 0;
     // w = DSq v_j = (D^dag D + fmass^2) v_j, as susy_dsq_dev
-    rc = dslash64(c, vj, c->d_tmp, 1, nullptr, 0.0, nullptr, nullptr);
-    if (!rc) rc = dslash64(c, c->d_tmp, w, -1, vj, f2, c->d_part, &nbd);
+    rc = dsq_pair64(c, vj, c->d_tmp, w, f2, c->d_part, &nbd);
     if (rc) break;
     double alpha;
     if ((rc = tf_dot_re(c, vj, w, &alpha))) break;

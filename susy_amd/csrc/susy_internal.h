@@ -190,6 +190,8 @@ struct susy_ctx {
   // operator scratch
   double2 *d_aux = nullptr;         // 12 complex per site (operator/force prep)
   float2 *f_aux = nullptr;
+  double2 *d_aux2 = nullptr;        // 6 complex per site: D^dagger prep written by D
+  float2 *f_aux2 = nullptr;
   double2 *d_tmp = nullptr;         // one TF (inner D of DSq)
   float2 *f_tmp = nullptr;
   double2 *d_boundary = nullptr;    // AoS staging for host transfers (one TF)
@@ -246,6 +248,10 @@ int dslash64(susy_ctx *c, const double2 *src, double2 *dst, int sign,
 int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
              const float2 *p0, float shift, double *partial, int *nblocks,
              const int *skip = nullptr);
+int dsq_pair64(susy_ctx *c, const double2 *src, double2 *tmp, double2 *dst, double shift,
+               double *partial, int *nblocks, const int *skip = nullptr);
+int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float shift,
+               double *partial, int *nblocks, const int *skip = nullptr);
 
 int fields_recompute(susy_ctx *c);      // det, Uinv, plaqdet, ZW*, DmuUmu, Fmunu
 int fields_refresh32(susy_ctx *c);      // FP32 copies of links, Uinv, plaqdet
