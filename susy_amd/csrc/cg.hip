@@ -248,19 +248,56 @@ k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, const CgState *__restric
 }
 
 // pout = r + alpha_0 pin (pout is pin, or the other p0 buffer at the first
-// iteration of a window, which leaves the window-start p0 intact)
+// iteration of a window, which leaves the window-start p0 intact), one thread
+// per (site, Twist_Fermion component); the thread also stores the
+// operator's pre-pass values of the new p for the next D:
+// aux[0] = Tr p_0, aux[1+a] = Tr(U_a^-1 p_{1+a}) (aux = nullptr: no G terms)
+template <typename C> struct ScalarOf;
+template <> struct ScalarOf<float2> { typedef float t; };
+template <> struct ScalarOf<double2> { typedef double t; };
 template <typename C>
 __global__ void __launch_bounds__(CB)
-k_cg_p0(const C *pin, const C *__restrict__ r, C *pout, const CgState *__restrict__ st,
-        size_t n) {
+k_cg_p0_prep(const C *pin, const C *__restrict__ r, C *pout, const C *__restrict__ uinv,
+             C *__restrict__ aux, const CgState *__restrict__ st, int V) {
+  typedef typename ScalarOf<C>::t R;
   if (st->done) return;
-  size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
-  if (i >= n) return;
+  const int s = blockIdx.x * CB + threadIdx.x, k = blockIdx.y;
+  if (s >= V) return;
   const double alpha0 = st->al[0];
-  C p = pin[i], rv = r[i];
-  p.x = rv.x + alpha0 * p.x;
-  p.y = rv.y + alpha0 * p.y;
-  pout[i] = p;
+  C v[NN];
+#pragma unroll
+  for (int e = 0; e < NN; e++) {
+    const size_t i = ((size_t)k * NN + e) * V + s;
+    C p = pin[i];
+    const C rv = r[i];
+    p.x = rv.x + alpha0 * p.x;
+    p.y = rv.y + alpha0 * p.y;
+    pout[i] = p;
+    v[e] = p;
+  }
+  if (aux == nullptr || k > NUMLINK) return;
+  R tx = 0, ty = 0;
+  if (k == 0) {
+#pragma unroll
+    for (int i = 0; i < NCOL; i++) {
+      tx += v[i * NCOL + i].x;
+      ty += v[i * NCOL + i].y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NCOL; i++)
+#pragma unroll
+      for (int j = 0; j < NCOL; j++) {
+        const C u = uinv[((size_t)(k - 1) * NN + i * NCOL + j) * V + s];
+        const C q = v[j * NCOL + i];
+        tx += u.x * q.x - u.y * q.y;
+        ty += u.x * q.y + u.y * q.x;
+      }
+  }
+  C t;
+  t.x = tx;
+  t.y = ty;
+  aux[(size_t)k * V + s] = t;
 }
 
 // Window flush.  Every touched system j updates psi_j and, for j >= 1, p_j:
@@ -391,6 +428,15 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   // the current p0 and pfrozen its value at the window start.
   C *r = slot(m - 1), *pcur = p0buf[0], *pfrozen = p0buf[0];
   int host_cnt = 0, since_reliable = 0;
+  // the p update also writes the operator pre-pass of the new p (G terms
+  // only); the first iteration's p (the source) gets the ordinary pre-pass
+  if (fp32 && !c->f_fresh) {
+    const int e0 = fields_refresh32(c);
+    if (e0) return e0;
+  }
+  C *aux_c = !c->doG ? nullptr : fp32 ? (C *)c->f_aux : (C *)c->d_aux;
+  const C *uinv_c = fp32 ? (const C *)c->f_uinv : (const C *)c->d_uinv;
+  bool p_prepped = false;
 
   const int nb = nblk(n);
   hipLaunchKernelGGL((k_cg_init<C>), dim3(nb), dim3(CB), 0, c->s, src, r, pcur, pj, nsh, psi,
@@ -428,14 +474,16 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
     pfrozen = pcur;
     return 0;
   };
-  auto dsq = [&](const C *in, C *out, int *nbo) -> int {
+  // in_prepped: the operator's pre-pass values of `in` are already in the
+  // context's aux buffer (written by k_cg_p0_prep)
+  auto dsq = [&](const C *in, C *out, int *nbo, bool in_prepped) -> int {
     int r1, nbd = 0;
     if (fp32) {
       r1 = dsq_pair32(c, (const float2 *)in, (float2 *)tmp, (float2 *)out, (float)sigma0,
-                      c->d_part, &nbd, d_done);
+                      c->d_part, &nbd, d_done, !in_prepped);
     } else {
       r1 = dsq_pair64(c, (const double2 *)in, (double2 *)tmp, (double2 *)out, sigma0,
-                      c->d_part, &nbd, d_done);
+                      c->d_part, &nbd, d_done, !in_prepped);
     }
     *nbo = nbd;
     return r1;
@@ -478,7 +526,7 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
       const Pos start = {r, pcur, pfrozen};
       for (int k = 0; k < CG_BATCH; k++) {
         int nbd;
-        if ((e = dsq(pcur, ap, &nbd))) return e;
+        if ((e = dsq(pcur, ap, &nbd, p_prepped))) return e;
         if ((e = reduce_partials(c, c->d_part, nbd, 1))) return e;
         hipLaunchKernelGGL(k_cgs_beta, dim3(1), dim3(1), 0, c->s, st, d_cd);
         C *rnew = slot(host_cnt);
@@ -488,9 +536,10 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
         if ((e = reduce_partials(c, c->d_part, nb, 2))) return e;
         hipLaunchKernelGGL(k_cgs_alpha, dim3(1), dim3(ALPHA_THREADS), 0, c->s, st, d_rn);
         C *pnext = pcur != pfrozen ? pcur : (pcur == p0buf[0] ? p0buf[1] : p0buf[0]);
-        hipLaunchKernelGGL((k_cg_p0<C>), dim3((n + CB - 1) / CB), dim3(CB), 0, c->s, pcur,
-                           rnew, pnext, st, n);
+        hipLaunchKernelGGL((k_cg_p0_prep<C>), dim3((c->V + CB - 1) / CB, NTF), dim3(CB), 0, c->s,
+                           pcur, rnew, pnext, uinv_c, aux_c, st, c->V);
         HIPCHK(hipGetLastError());
+        p_prepped = aux_c != nullptr;
         r = rnew;
         pcur = pnext;
         host_cnt++;

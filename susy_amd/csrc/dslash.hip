@@ -559,23 +559,24 @@ int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
 // dst = D^dagger D src + shift src, tmp = D src, per-block partials of
 // Re <src, dst> in partial (the DSq of utilities.c:874-885 with the CG's
 // shift).  D stores the D^dagger pre-pass values of its output (aux2), so
-// D^dagger runs without its pre-pass.
+// D^dagger runs without its pre-pass.  prep_src = false: the caller has
+// already stored the pre-pass values of src in the context's aux buffer.
 int dsq_pair64(susy_ctx *c, const double2 *src, double2 *tmp, double2 *dst, double shift,
-               double *partial, int *nblocks, const int *skip) {
+               double *partial, int *nblocks, const int *skip, bool prep_src) {
   int r = dslash_launch<double>(c, src, tmp, 1, nullptr, 0.0, nullptr, nullptr, c->d_links,
-                                c->d_uinv, c->d_pdet, c->d_aux, skip, c->d_aux2);
+                                c->d_uinv, c->d_pdet, c->d_aux, skip, c->d_aux2, prep_src);
   if (r) return r;
   return dslash_launch<double>(c, tmp, dst, -1, src, shift, partial, nblocks, c->d_links,
                                c->d_uinv, c->d_pdet, c->d_aux2, skip, nullptr, false);
 }
 int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float shift,
-               double *partial, int *nblocks, const int *skip) {
+               double *partial, int *nblocks, const int *skip, bool prep_src) {
   if (!c->f_fresh) {
     int r = fields_refresh32(c);
     if (r) return r;
   }
   int r = dslash_launch<float>(c, src, tmp, 1, nullptr, 0.f, nullptr, nullptr, c->f_links,
-                               c->f_uinv, c->f_pdet, c->f_aux, skip, c->f_aux2);
+                               c->f_uinv, c->f_pdet, c->f_aux, skip, c->f_aux2, prep_src);
   if (r) return r;
   return dslash_launch<float>(c, tmp, dst, -1, src, shift, partial, nblocks, c->f_links,
                               c->f_uinv, c->f_pdet, c->f_aux2, skip, nullptr, false);

@@ -249,9 +249,9 @@ int dslash32(susy_ctx *c, const float2 *src, float2 *dst, int sign,
              const float2 *p0, float shift, double *partial, int *nblocks,
              const int *skip = nullptr);
 int dsq_pair64(susy_ctx *c, const double2 *src, double2 *tmp, double2 *dst, double shift,
-               double *partial, int *nblocks, const int *skip = nullptr);
+               double *partial, int *nblocks, const int *skip = nullptr, bool prep_src = true);
 int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float shift,
-               double *partial, int *nblocks, const int *skip = nullptr);
+               double *partial, int *nblocks, const int *skip = nullptr, bool prep_src = true);
 
 int fields_recompute(susy_ctx *c);      // det, Uinv, plaqdet, ZW*, DmuUmu, Fmunu
 int fields_refresh32(susy_ctx *c);      // FP32 copies of links, Uinv, plaqdet
