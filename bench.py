@@ -17,10 +17,16 @@ iterations per solve), so `--traj-length 1 --nstep 20` is available but not
 the default.
 
 Forms: the headline FP32 form (FP32 operator and multi-shift CG with FP64
-reliable updates, FP64 forces, actions and dH) and the plain FP64 form, both
-from the same links with the same generator seed (identical momenta and
-pseudofermions), so the first trajectories of the two forms differ only by
-arithmetic.
+reliable updates, FP64 forces, actions and dH), the plain FP64 form and the
+FP32 form with the learned surrogate force on the inner MD level, all from
+the same links with the same generator seed (identical momenta and
+pseudofermions), so their first trajectories differ only by arithmetic and
+by the surrogate.  The surrogate is trained in the run itself: the plain
+FP64 trajectory records (links, fermion force) at every outer kick, the
+network is trained on them with the library's own forward and backward
+kernels (Adam, `--sg-steps`), and its relative force error on the last two
+recorded pairs (held out) is reported with the form.  The surrogate form is
+timed but is not the headline `value`: it is reported beside it.
 
 `--kernels` runs only a short operator workload (5 field copies for the
 memory-counter calibration, then 10 launches of the operator in each
@@ -162,6 +168,8 @@ def main():
                     help="time every n-th operator kernel launch with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="operator launches only")
+    ap.add_argument("--sg-steps", type=int, default=300,
+                    help="Adam steps of the surrogate training (0: no surrogate form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -195,8 +203,30 @@ def main():
 
     st32, t32, prof32, plaq32 = run_form(lat, susy_amd.FORM_FP32, args.steps, start,
                                          args.prof_every)
+    if args.sg_steps > 0:
+        # the plain run records the (links, fermion force) pairs
+        lat.sg_init(seed=1, scale=1.0)
+        lat.sg_record(4 * (args.nstep + 2) * max(args.fp64_steps, 1))
     st64, t64, prof64, plaq64 = run_form(lat, susy_amd.FORM_FP64, args.fp64_steps, start,
                                          args.prof_every)
+    sg = None
+    if args.sg_steps > 0:
+        lat.sg_record(0)
+        npairs = lat.sg_pairs()
+        ntest = 2 if npairs > 4 else 0
+        held = list(range(npairs - ntest, npairs))
+        before = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
+        t = time.perf_counter()
+        hist = lat.sg_train(0, npairs - ntest, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
+        t_train = time.perf_counter() - t
+        after = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
+        stsg, tsg, profsg, plaqsg = run_form(lat, susy_amd.FORM_FP32 | susy_amd.FORM_SURROGATE,
+                                             args.steps, start, args.prof_every)
+        sg = (stsg, tsg, profsg, plaqsg, {
+            "pairs": npairs, "held_out": ntest, "train_steps": args.sg_steps,
+            "train_s": t_train, "train_loss_first_last": [float(np.mean(hist[:10])),
+                                                          float(np.mean(hist[-10:]))],
+            "held_out_force_err2_untrained": before, "held_out_force_err2_trained": after})
 
     def summary(stats, secs, prof, plaq, nbytes):
         acc = sum(s[3] for s in stats)
@@ -215,6 +245,14 @@ def main():
 
     f32 = summary(st32, t32, prof32, plaq32, 8)
     f64 = summary(st64, t64, prof64, plaq64, 16)
+    forms = {"fp32": f32, "fp64": f64,
+             "speedup_fp32_vs_fp64": f64["s_per_traj"] / f32["s_per_traj"],
+             "dH_abs_diff_first_traj": abs(st32[0][2] - st64[0][2])}
+    if sg is not None:
+        fsg = summary(sg[0], sg[1], sg[2], sg[3], 8)
+        fsg["training"] = sg[4]
+        forms["fp32_surrogate"] = fsg
+        forms["speedup_fp32_surrogate_vs_fp64"] = f64["s_per_traj"] / fsg["s_per_traj"]
     nforce_solves = args.nstep * 2 + 1
     bytes32 = dslash_bytes(V, 8)
     avg_ms = prof32[0]
@@ -233,16 +271,16 @@ def main():
                    "nstep": args.nstep, "nstep_gauge": args.nstep_gauge,
                    "form": "FP32 operator and CG with FP64 reliable updates; FP64 forces, "
                            "actions and dH",
-                   "surrogate": "not used (the learned force is not built)"},
+                   "surrogate": "value is the FP32 form; the FP32+surrogate form (trained "
+                                "in this run on the FP64 trajectory's force pairs) is "
+                                "reported under forms.fp32_surrogate"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(),
                      "kernel": "k_dslash<float, false, 0>: D = fermion_op(sign +1), FP32",
                      "launches": prof32[1], "sampled": prof32[2], "avg_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes32},
-        "forms": {"fp32": f32, "fp64": f64,
-                  "speedup_fp32_vs_fp64": f64["s_per_traj"] / f32["s_per_traj"],
-                  "dH_abs_diff_first_traj": abs(st32[0][2] - st64[0][2])},
+        "forms": forms,
         "dslash_isolated": {"fp64_ms": iso["fp64"],
                             "fp64_GBs": dslash_bytes(V, 16) / iso["fp64"] / 1e6,
                             "fp32_ms": iso["fp32"], "fp32_GBs": bytes32 / iso["fp32"] / 1e6},

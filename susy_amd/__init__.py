@@ -405,7 +405,7 @@ class Lattice:
         return it.value
 
 
-def set_cg_options(ncol, window=32, reliable=50):
+def set_cg_options(ncol, window=32, reliable=128):
     r = load(ncol).susy_set_cg_options(window, reliable)
     if r < 0:
         raise RuntimeError(load(ncol).susy_last_error().decode())

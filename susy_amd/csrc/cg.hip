@@ -62,6 +62,7 @@ __device__ __forceinline__ void cg_block_sum(double v, double *partial) {
     double t = 0;
     for (int w = 0; w < CB / 64; w++) t += red[w];
     partial[blockIdx.x] = t;
+    hold(t);
   }
 }
 
@@ -77,11 +78,12 @@ k_cg_init(const double2 *__restrict__ src, C *__restrict__ r, C *__restrict__ p0
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
     double2 v = src[i];
+    vm_drain();
     C w = from64<C>(v);
-    r[i] = w;
-    p0[i] = w;
-    for (int j = 1; j < nsh; j++) pj[(size_t)(j - 1) * n + i] = w;
-    for (int j = 0; j < nsh; j++) psi.p[j][i] = make_double2(0, 0);
+    st_keep(r + i, w);
+    st_keep(p0 + i, w);
+    for (int j = 1; j < nsh; j++) st_keep(pj + (size_t)(j - 1) * n + i, w);
+    for (int j = 0; j < nsh; j++) st_keep(psi.p[j] + i, make_double2(0, 0));
     s += v.x * v.x + v.y * v.y;
   }
   cg_block_sum(s, partial);
@@ -106,23 +108,37 @@ struct Shifts {
 
 // Arithmetic of the scalar kernels is not contracted into FMAs, so it rounds
 // like the reference's host code.
-__global__ void k_cgs_init(CgState *st, const double *__restrict__ source_norm,
-                           double errormin, int nsh, Shifts sh, int maxcg) {
+// Initial state, one thread per window coefficient (MAXSH x MAXWIN threads):
+// few stores per thread, so no store's data register is reused behind it.
+#define STATE_THREADS (MAXSH * MAXWIN)
+__global__ void __launch_bounds__(STATE_THREADS)
+k_cgs_init(CgState *st, const double *__restrict__ source_norm, double errormin, int nsh,
+           Shifts sh, int maxcg) {
 #pragma clang fp contract(off)
-  st->rsq = *source_norm;
-  st->rsqstop = errormin * st->rsq;
-  st->iteration = 0;
-  st->maxcg = maxcg;
-  st->nsh = nsh;
-  st->cnt = 0;
-  st->done = !(0 < maxcg && st->rsq > st->rsqstop);
-  st->skip_rel = 0;
-  for (int j = 0; j < MAXSH; j++) {
-    st->shifts[j] = j < nsh ? sh.v[j] : 0.0;
+  const int j = threadIdx.x / MAXWIN, l = threadIdx.x % MAXWIN;
+  st->Cw[j][l] = 0;
+  st->Dw[j][l] = 0;
+  if (l == 0) {
+    const double shj = j < nsh ? sh.v[j] : 0.0;
+    st->shifts[j] = shj;
     st->zim1[j] = 1; st->zi[j] = 1; st->al[j] = 0; st->bim1[j] = 1; st->conv[j] = 0;
     st->zip1[j] = 1; st->bi[j] = 0;
     st->A[j] = 1; st->E[j] = 0; st->touched[j] = 0;
-    for (int l = 0; l < MAXWIN; l++) st->Cw[j][l] = st->Dw[j][l] = 0;
+    hold(shj);
+  }
+  if (threadIdx.x == 0) {
+    const double rsq = *source_norm;
+    const double stop = errormin * rsq;
+    st->rsq = rsq;
+    st->rsqstop = stop;
+    st->iteration = 0;
+    st->maxcg = maxcg;
+    st->nsh = nsh;
+    st->cnt = 0;
+    st->done = !(0 < maxcg && rsq > stop);
+    st->skip_rel = 0;
+    hold(rsq);
+    hold(stop);
   }
 }
 
@@ -138,19 +154,30 @@ __global__ void k_cgs_stop(CgState *st) {
 __global__ void k_cgs_beta(CgState *st, const double *__restrict__ cd) {
 #pragma clang fp contract(off)
   if (st->done) return;
-  st->iteration++;
-  double *zi = st->zi, *zim1 = st->zim1, *zip1 = st->zip1, *bi = st->bi, *bim1 = st->bim1,
-         *al = st->al;
-  const double *shifts = st->shifts;
-  bi[0] = -st->rsq / *cd;
-  zip1[0] = 1;
-  for (int j = 1; j < st->nsh; j++) {
-    if (st->conv[j]) continue;
-    zip1[j] = zi[j] * zim1[j] * bim1[0];
-    double c1 = bi[0] * al[0] * (zim1[j] - zi[j]);
-    double c2 = zim1[j] * bim1[0] * (1 - (shifts[j] - shifts[0]) * bi[0]);
-    zip1[j] /= c1 + c2;
-    bi[j] = bi[0] * zip1[j] / zi[j];
+  // every load before the first store (vm_drain: see its definition)
+  const int it = st->iteration, nsh = st->nsh;
+  const double rsq = st->rsq, cdv = *cd, al0 = st->al[0], bim10 = st->bim1[0],
+               s0 = st->shifts[0];
+  vm_drain();
+  const double bi0 = -rsq / cdv;
+  st->iteration = it + 1;
+  st->bi[0] = bi0;
+  hold(bi0);
+  st->zip1[0] = 1;
+  for (int j = 1; j < nsh; j++) {
+    const int cv = st->conv[j];
+    const double zij = st->zi[j], zim1j = st->zim1[j], sj = st->shifts[j];
+    vm_drain();
+    if (cv) continue;
+    double zp = zij * zim1j * bim10;
+    const double c1 = bi0 * al0 * (zim1j - zij);
+    const double c2 = zim1j * bim10 * (1 - (sj - s0) * bi0);
+    zp /= c1 + c2;
+    const double bj = bi0 * zp / zij;
+    st->zip1[j] = zp;
+    st->bi[j] = bj;
+    hold(zp);
+    hold(bj);
   }
 }
 
@@ -168,37 +195,67 @@ k_cgs_alpha(CgState *st, const double *__restrict__ rsqnew) {
   __shared__ double al_s[MAXSH], bi_s[MAXSH];
   __shared__ int conv_s[MAXSH];
   const int j = threadIdx.x / MAXWIN, l = threadIdx.x % MAXWIN;
+  // every load before the first store (vm_drain: see its definition)
   const int nsh = st->nsh, cnt = st->cnt;
-  const double rn = *rsqnew, rsq = st->rsq;
-  double *zi = st->zi, *zim1 = st->zim1, *zip1 = st->zip1, *bi = st->bi, *bim1 = st->bim1,
-         *al = st->al;
-  if (l == 0 && j < nsh) {
-    const int cv = st->conv[j];
+  const double rn = *rsqnew, rsq = st->rsq, rsqstop = st->rsqstop;
+  const bool lead = l == 0 && j < nsh;
+  int cv = 0;
+  double zij = 0, zp1 = 0, bij = 0, bi0 = 0, alj = 0, Ej = 0, Aj = 0, cw = 0, dw = 0;
+  if (lead) {
+    cv = st->conv[j];
+    zij = st->zi[j];
+    zp1 = st->zip1[j];
+    bij = st->bi[j];
+    bi0 = st->bi[0];
+    alj = st->al[j];
+    Ej = st->E[j];
+    Aj = st->A[j];
+  }
+  if (j < nsh && l < cnt) {
+    cw = st->Cw[j][l];
+    dw = st->Dw[j][l];
+  }
+  vm_drain();
+  if (lead) {
     const double a0 = rn / rsq;
-    if (j == 0) al[0] = a0;
-    else if (!cv) al[j] = a0 * zip1[j] * bi[j] / (zi[j] * bi[0]);
-    al_s[j] = al[j];
-    bi_s[j] = bi[j];
+    if (j == 0) alj = a0;
+    else if (!cv) alj = a0 * zp1 * bij / (zij * bi0);
+    al_s[j] = alj;
+    bi_s[j] = bij;
     conv_s[j] = cv;
   }
-  __syncthreads();   // every thread has read cnt, rsq and the old conv
+  __syncthreads();
   if (j < nsh && !conv_s[j]) {
     // psi_j -= beta_j p_j, then p_j = zeta_j r + alpha_j p_j (zeta_0 = 1)
     if (l < cnt) {
-      st->Dw[j][l] -= bi_s[j] * st->Cw[j][l];
-      st->Cw[j][l] *= al_s[j];
+      const double d2 = dw - bi_s[j] * cw, c2 = cw * al_s[j];
+      st->Dw[j][l] = d2;
+      st->Cw[j][l] = c2;
+      hold(d2);
+      hold(c2);
     }
     if (l == 0) {
+      const double e2 = Ej - bij * Aj, a2 = Aj * alj;
+      st->al[j] = alj;
       st->touched[j] = 1;
-      st->E[j] -= bi_s[j] * st->A[j];
-      st->A[j] *= al_s[j];
-      st->Cw[j][cnt] = zip1[j];
+      st->E[j] = e2;
+      st->A[j] = a2;
+      st->Cw[j][cnt] = zp1;
+      hold(e2);
+      hold(a2);
     }
   }
-  if (l == 0 && j < nsh) {
-    int cv = conv_s[j];
-    if (j > 0 && !cv && rn * zip1[j] * zip1[j] <= st->rsqstop) cv = st->conv[j] = 1;
-    if (!cv) { bim1[j] = bi_s[j]; zim1[j] = zi[j]; zi[j] = zip1[j]; }
+  if (lead) {
+    int c2 = cv;
+    if (j > 0 && !cv && rn * zp1 * zp1 <= rsqstop) {
+      c2 = 1;
+      st->conv[j] = 1;
+    }
+    if (!c2) {
+      st->bim1[j] = bij;
+      st->zim1[j] = zij;
+      st->zi[j] = zp1;
+    }
   }
   if (threadIdx.x == 0) {
     st->cnt = cnt + 1;
@@ -239,9 +296,10 @@ k_cg_r(const C *rin, const C *__restrict__ ap, C *rout, const CgState *__restric
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
     C a = ap[i], v = rin[i];
+    vm_drain();   // loads of this element complete before its store
     v.x = fma(beta0, a.x, v.x);
     v.y = fma(beta0, a.y, v.y);
-    rout[i] = v;
+    st_keep(rout + i, v);
     s += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
   }
   cg_block_sum(s, partial);
@@ -264,18 +322,29 @@ k_cg_p0_prep(const C *pin, const C *__restrict__ r, C *pout, const C *__restrict
   const int s = blockIdx.x * CB + threadIdx.x, k = blockIdx.y;
   if (s >= V) return;
   const double alpha0 = st->al[0];
-  C v[NN];
+  const bool pre = aux != nullptr && k <= NUMLINK;
+  // every load before the first store (vm_drain: see its definition)
+  C v[NN], rv[NN], u[NN];
 #pragma unroll
   for (int e = 0; e < NN; e++) {
     const size_t i = ((size_t)k * NN + e) * V + s;
-    C p = pin[i];
-    const C rv = r[i];
-    p.x = rv.x + alpha0 * p.x;
-    p.y = rv.y + alpha0 * p.y;
-    pout[i] = p;
+    v[e] = pin[i];
+    rv[e] = r[i];
+  }
+  if (pre && k > 0) {
+#pragma unroll
+    for (int e = 0; e < NN; e++) u[e] = uinv[((size_t)(k - 1) * NN + e) * V + s];
+  }
+  vm_drain();
+#pragma unroll
+  for (int e = 0; e < NN; e++) {
+    C p = v[e];
+    p.x = rv[e].x + alpha0 * p.x;
+    p.y = rv[e].y + alpha0 * p.y;
+    st_keep(pout + ((size_t)k * NN + e) * V + s, p);
     v[e] = p;
   }
-  if (aux == nullptr || k > NUMLINK) return;
+  if (!pre) return;
   R tx = 0, ty = 0;
   if (k == 0) {
 #pragma unroll
@@ -288,16 +357,15 @@ k_cg_p0_prep(const C *pin, const C *__restrict__ r, C *pout, const C *__restrict
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
       for (int j = 0; j < NCOL; j++) {
-        const C u = uinv[((size_t)(k - 1) * NN + i * NCOL + j) * V + s];
-        const C q = v[j * NCOL + i];
-        tx += u.x * q.x - u.y * q.y;
-        ty += u.x * q.y + u.y * q.x;
+        const C uu = u[i * NCOL + j], q = v[j * NCOL + i];
+        tx += uu.x * q.x - uu.y * q.y;
+        ty += uu.x * q.y + uu.y * q.x;
       }
   }
   C t;
   t.x = tx;
   t.y = ty;
-  aux[(size_t)k * V + s] = t;
+  st_keep(aux + (size_t)k * V + s, t);
 }
 
 // Window flush.  Every touched system j updates psi_j and, for j >= 1, p_j:
@@ -312,13 +380,15 @@ template <typename C, int M>
 __global__ void __launch_bounds__(CB)
 k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__restrict__ st,
            const int *__restrict__ skip, size_t n) {
-  if ((skip != nullptr && *skip) || st->cnt == 0) return;
+  const int cnt = st->cnt;
+  if ((skip != nullptr && *skip) || cnt == 0) return;
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
-  // all M slots are read; slots beyond this window's count carry zero weights
+  // only this window's slots are read (the others may hold data of another
+  // solve, in either precision)
   double2 w[M];
 #pragma unroll
-  for (int l = 0; l < M; l++) w[l] = to64(win[(size_t)l * n + i]);
+  for (int l = 0; l < M; l++) w[l] = l < cnt ? to64(win[(size_t)l * n + i]) : make_double2(0, 0);
   const int nsh = st->nsh;
   for (int j = 0; j < nsh; j++) {
     if (!st->touched[j]) continue;
@@ -326,6 +396,7 @@ k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__re
     double2 p = to64(*pk);
     double2 x = psi.p[j][i];
     const double A = st->A[j], E = st->E[j];
+    vm_drain();   // this shift's loads complete before its stores
     double2 np = make_double2(A * p.x, A * p.y);
     x.x += E * p.x;
     x.y += E * p.y;
@@ -337,8 +408,8 @@ k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__re
       x.x += dl * w[l].x;
       x.y += dl * w[l].y;
     }
-    if (j > 0) *pk = from64<C>(np);
-    psi.p[j][i] = x;
+    if (j > 0) st_keep(pk, from64<C>(np));
+    st_keep(psi.p[j] + i, x);
   }
 }
 
@@ -352,8 +423,9 @@ k_true_res(const double2 *__restrict__ src, const double2 *__restrict__ ap,
   double s = 0;
   for (size_t i = (size_t)blockIdx.x * CB + threadIdx.x; i < n; i += (size_t)gridDim.x * CB) {
     double2 a = src[i], b = ap[i];
+    vm_drain();
     double2 v = make_double2(a.x - b.x, a.y - b.y);
-    r[i] = from64<C>(v);
+    st_keep(r + i, from64<C>(v));
     s += v.x * v.x + v.y * v.y;
   }
   cg_block_sum(s, partial);
@@ -366,7 +438,9 @@ static int nblk(size_t n) {
 }
 
 static int g_window = 32;     // residuals kept per window (<= MAXWIN)
-static int g_reliable = 50;   // FP32 form: iterations between true residuals
+// FP32 form: iterations between true residuals; a multiple of the window, so
+// the reliable update's flush finds the window already flushed
+static int g_reliable = 128;
 
 extern "C" int susy_set_cg_options(int window, int reliable) {
   if ((window != 1 && window != 4 && window != 8 && window != 16 && window != 32) ||
@@ -446,8 +520,8 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
   if (e) return e;
   Shifts shv;
   for (int j = 0; j < MAXSH; j++) shv.v[j] = j < nsh ? shifts[j] : 0.0;
-  hipLaunchKernelGGL(k_cgs_init, dim3(1), dim3(1), 0, c->s, st, d_norm, errormin, nsh, shv,
-                     MaxCG);
+  hipLaunchKernelGGL(k_cgs_init, dim3(1), dim3(STATE_THREADS), 0, c->s, st, d_norm, errormin, nsh,
+                     shv, MaxCG);
   HIPCHK(hipGetLastError());
 
   auto one = [&](void (*k)(CgState *)) -> int {

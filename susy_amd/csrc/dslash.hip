@@ -83,13 +83,16 @@ __device__ __forceinline__ int bload_i32(Rsrc r, uint32_t voff, uint32_t soff) {
   return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 template <typename C> __device__ __forceinline__ void bstore(C v, Rsrc r, uint32_t voff, uint32_t soff);
+// (the data stays live until the store has read it: see st_keep)
 template <>
 __device__ __forceinline__ void bstore<double2>(double2 v, Rsrc r, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, voff, soff, 0);
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
 }
 template <>
 __device__ __forceinline__ void bstore<float2>(float2 v, Rsrc r, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, voff, soff, 0);
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
 }
 
 // A matrix operand: field descriptor, lane byte offset of the site, uniform
@@ -142,19 +145,19 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
   typedef typename CT<R>::t C;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= V || (skip != nullptr && *skip)) return;
-  C tr = mkc<C>(0, 0);
+  C t[1 + NUMLINK];
+  t[0] = mkc<C>(0, 0);
 #pragma unroll
   for (int i = 0; i < NCOL; i++) {
     C v = src[(i * NCOL + i) * V + s];
-    tr.x += v.x;
-    tr.y += DAG ? -v.y : v.y;
+    t[0].x += v.x;
+    t[0].y += DAG ? -v.y : v.y;
   }
-  aux[s] = tr;
 #pragma unroll
   for (int a = 0; a < NUMLINK; a++) {
     const C *psi = src + (size_t)(1 + a) * NN * V + s;
     const C *ui = Uinv + (size_t)a * NN * V + s;
-    C t = mkc<C>(0, 0);
+    C ta = mkc<C>(0, 0);
 #pragma unroll
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
@@ -162,11 +165,14 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
         C u = ui[(i * NCOL + j) * V];
         C p = DAG ? psi[(i * NCOL + j) * V] : psi[(j * NCOL + i) * V];
         if (DAG) p.y = -p.y;
-        t.x += u.x * p.x - u.y * p.y;
-        t.y += u.x * p.y + u.y * p.x;
+        ta.x += u.x * p.x - u.y * p.y;
+        ta.y += u.x * p.y + u.y * p.x;
       }
-    aux[(size_t)(1 + a) * V + s] = t;
+    t[1 + a] = ta;
   }
+  vm_drain();
+#pragma unroll
+  for (int a = 0; a <= NUMLINK; a++) st_keep(aux + (size_t)a * V + s, t[a]);
 }
 
 #define DS_THREADS (DS_SITES * DS_SPLIT)
@@ -388,9 +394,28 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
 
   double dot = 0.0;
   if (live && first) {
-    // ---- store (D^dagger: negated adjoint)
+    // ---- epilogue loads (p0 for the DSq epilogue, U^-1 for the D^dagger
+    // pre-pass values), all drained before the first store
     const Rsrc rD = make_rsrc(dst, NTF * Mb);
-    const Rsrc rQ = make_rsrc(EPI == 1 ? p0 : dst, NTF * Mb);
+    const bool pre = !DAG && EPI == 0 && aux_out != nullptr && comp <= NUMLINK;
+    C q[NCOL][NCOL];
+    if (EPI == 1) {
+      const Rsrc rQ = make_rsrc(p0, NTF * Mb);
+#pragma unroll
+      for (int i = 0; i < NCOL; i++)
+#pragma unroll
+        for (int k = 0; k < NCOL; k++)
+          q[i][k] = bload<C>(rQ, (uint32_t)s * cs, comp * Mb + (i * NCOL + k) * pl);
+    } else if (pre && comp > 0) {
+      const Rsrc rI = make_rsrc(Uinv, NUMLINK * Mb);
+#pragma unroll
+      for (int i = 0; i < NCOL; i++)
+#pragma unroll
+        for (int k = 0; k < NCOL; k++)
+          q[i][k] = bload<C>(rI, (uint32_t)s * cs, (comp - 1) * Mb + (i * NCOL + k) * pl);
+    }
+    vm_drain();
+    // ---- store (D^dagger: negated adjoint)
 #pragma unroll
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
@@ -398,14 +423,13 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
         C v = DAG ? mkc<C>(-acc[k][i].x, acc[k][i].y) : acc[i][k];
         const uint32_t so = comp * Mb + (i * NCOL + k) * pl;
         if (EPI == 1) {
-          const C q = bload<C>(rQ, (uint32_t)s * cs, so);
-          v.x = fma(shift, q.x, v.x);
-          v.y = fma(shift, q.y, v.y);
-          dot += (double)q.x * (double)v.x + (double)q.y * (double)v.y;
+          v.x = fma(shift, q[i][k].x, v.x);
+          v.y = fma(shift, q[i][k].y, v.y);
+          dot += (double)q[i][k].x * (double)v.x + (double)q[i][k].y * (double)v.y;
         }
         bstore<C>(v, rD, (uint32_t)s * cs, so);
       }
-    if (!DAG && EPI == 0 && aux_out != nullptr && comp <= NUMLINK) {
+    if (pre) {
       C tr = mkc<C>(0, 0);
       if (comp == 0) {
 #pragma unroll
@@ -414,18 +438,16 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
           tr.y -= acc[i][i].y;
         }
       } else {
-        const Rsrc rI = make_rsrc(Uinv, NUMLINK * Mb);
 #pragma unroll
         for (int i = 0; i < NCOL; i++)
 #pragma unroll
           for (int j = 0; j < NCOL; j++) {
-            const C u = bload<C>(rI, (uint32_t)s * cs, (comp - 1) * Mb + (i * NCOL + j) * pl);
-            const C p = acc[i][j];
+            const C u = q[i][j], p = acc[i][j];
             tr.x += u.x * p.x + u.y * p.y;
             tr.y += u.y * p.x - u.x * p.y;
           }
       }
-      aux_out[(size_t)comp * V + s] = tr;
+      st_keep(aux_out + (size_t)comp * V + s, tr);
     }
   }
   if (EPI == 1) {

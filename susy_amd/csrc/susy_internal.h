@@ -80,6 +80,28 @@ inline int ds_grid(int V, int tslice, int paired, DsOrder *o) {
 
 // ---------------------------------------------------------------------
 // Complex helpers for double2 / float2
+// Completes every outstanding vector-memory access of the wave and keeps the
+// compiler from moving memory accesses across it: the kernels issue all
+// their loads, drain, then store.
+__device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// A vector-memory store reads its data VGPRs some cycles after it issues.
+// On this hardware an instruction that overwrites those VGPRs right after
+// the store (which the compiler schedules freely) can win the race, and the
+// store then writes the new value -- seen as sporadic wrong 64-byte groups
+// of the operator's FP64 output at U(3) 8^3x16 and up, never at 4^4.  These
+// stores keep the data live until s_waitcnt expcnt(0) reports it read.
+__device__ __forceinline__ void hold(double v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
+__device__ __forceinline__ void hold(float v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
+__device__ __forceinline__ void st_keep(double2 *p, double2 v) {
+  *p = v;
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+}
+__device__ __forceinline__ void st_keep(float2 *p, float2 v) {
+  *p = v;
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+}
+
 template <typename R> struct CT;
 template <> struct CT<double> { typedef double2 t; };
 template <> struct CT<float> { typedef float2 t; };

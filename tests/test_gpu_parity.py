@@ -149,7 +149,7 @@ def test_congrad_multi_fp32_reliable_at_convergence(offset):
             psi = lat.tf_download(hs + 1 + j)
             r = src - (lat.tf_download(hs + 16) + shifts[j] * psi)
             assert np.vdot(r, r).real <= 4 * err * src_n2, (window, j)
-    susy_amd.set_cg_options(2, window=32, reliable=50)
+    susy_amd.set_cg_options(2)
 
 
 @pytest.mark.parametrize("ncol", [2, 3, 4])
