@@ -341,9 +341,11 @@ k_cg_p0_prep(const C *pin, const C *__restrict__ r, C *pout, const C *__restrict
     C p = v[e];
     p.x = rv[e].x + alpha0 * p.x;
     p.y = rv[e].y + alpha0 * p.y;
-    st_keep(pout + ((size_t)k * NN + e) * V + s, p);
+    pout[((size_t)k * NN + e) * V + s] = p;
     v[e] = p;
   }
+#pragma unroll
+  for (int e = 0; e < NN; e++) hold(v[e]);
   if (!pre) return;
   R tx = 0, ty = 0;
   if (k == 0) {

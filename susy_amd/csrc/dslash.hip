@@ -83,16 +83,15 @@ __device__ __forceinline__ int bload_i32(Rsrc r, uint32_t voff, uint32_t soff) {
   return (int)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 template <typename C> __device__ __forceinline__ void bstore(C v, Rsrc r, uint32_t voff, uint32_t soff);
-// (the data stays live until the store has read it: see st_keep)
+// (callers keep the stored values live until the stores have read them:
+// hold() after the last store, see st_keep)
 template <>
 __device__ __forceinline__ void bstore<double2>(double2 v, Rsrc r, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, voff, soff, 0);
-  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
 }
 template <>
 __device__ __forceinline__ void bstore<float2>(float2 v, Rsrc r, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, voff, soff, 0);
-  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
 }
 
 // A matrix operand: field descriptor, lane byte offset of the site, uniform
@@ -415,7 +414,9 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
           q[i][k] = bload<C>(rI, (uint32_t)s * cs, (comp - 1) * Mb + (i * NCOL + k) * pl);
     }
     vm_drain();
-    // ---- store (D^dagger: negated adjoint)
+    // ---- store (D^dagger: negated adjoint); the stored values stay live
+    // until one wait after the last store reports their registers read
+    C out[NCOL][NCOL];
 #pragma unroll
     for (int i = 0; i < NCOL; i++)
 #pragma unroll
@@ -428,7 +429,12 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
           dot += (double)q[i][k].x * (double)v.x + (double)q[i][k].y * (double)v.y;
         }
         bstore<C>(v, rD, (uint32_t)s * cs, so);
+        out[i][k] = v;
       }
+#pragma unroll
+    for (int i = 0; i < NCOL; i++)
+#pragma unroll
+      for (int k = 0; k < NCOL; k++) hold(out[i][k]);
     if (pre) {
       C tr = mkc<C>(0, 0);
       if (comp == 0) {

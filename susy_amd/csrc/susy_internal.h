@@ -93,6 +93,12 @@ __device__ __forceinline__ void vm_drain() { asm volatile("s_waitcnt vmcnt(0)" :
 // stores keep the data live until s_waitcnt expcnt(0) reports it read.
 __device__ __forceinline__ void hold(double v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
 __device__ __forceinline__ void hold(float v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
+__device__ __forceinline__ void hold(double2 v) {
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+}
+__device__ __forceinline__ void hold(float2 v) {
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+}
 __device__ __forceinline__ void st_keep(double2 *p, double2 v) {
   *p = v;
   asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
