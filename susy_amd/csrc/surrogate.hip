@@ -63,6 +63,7 @@ struct SgState {
   float *theta = nullptr, *adam_m = nullptr, *adam_v = nullptr;
   unsigned short *wbf = nullptr;                // bfloat16 copies (same layout)
   double *gacc = nullptr;                       // gradient accumulator
+  double *wpart = nullptr;                      // per-chunk weight-gradient partials
   float *phi = nullptr, *h1 = nullptr, *h2 = nullptr, *gz2 = nullptr, *gz1 = nullptr,
         *gphi = nullptr;
   float *v = nullptr, *z1d = nullptr, *h1d = nullptr, *h2d = nullptr, *az2d = nullptr,
@@ -545,41 +546,103 @@ k_sg_tan3(const unsigned short *__restrict__ wbf, const float *__restrict__ az2d
   }
 }
 
-// parameter gradients of the tangent, one thread per parameter, summed over
-// sites in a fixed order and added to gacc
+// parameter gradients of the tangent, summed over sites and added to gacc
 struct SgGradIn {
   const float *phi, *v, *h1, *h1d, *h2d, *az2d, *az2, *az1d, *az1;
 };
+// Two passes.  k_sg_wpart: block (chunk of WG_CHUNK sites, tap t) stages
+// WG_SUB sites at a time in LDS -- the adjoints at x and the layer inputs at
+// x + d_t -- and accumulates the tap's 1040 + 256 weight entries (thread
+// p, p + 256, ...) over its chunk; the tap-0 blocks also sum the 48 bias
+// and w3 entries.  k_sg_wsum adds the chunk partials of each parameter in
+// chunk order (deterministic).
+#define WG_SUB 64
+#define WG_CHUNK 1024
+#define WG_P1 (SG_NC * SG_NF)
+#define WG_PT (WG_P1 + SG_NC * SG_NC)
+#define WG_NB (3 * SG_NC)
+#define WG_SLOTS ((WG_PT + SGB - 1) / SGB)
 __global__ void __launch_bounds__(SGB)
-k_sg_wgrad(SgGradIn in, const int *__restrict__ nbr, double *__restrict__ gacc, int V,
-           double scale) {
+k_sg_wpart(SgGradIn in, const int *__restrict__ nbr, double *__restrict__ part,
+           double *__restrict__ bpart, int V) {
+  __shared__ float A1[2][WG_SUB][SG_NC], B1[2][WG_SUB][SG_NF];
+  __shared__ float A2[2][WG_SUB][SG_NC], B2[2][WG_SUB][SG_NC];
+  __shared__ int ys[WG_SUB];
+  const int chunk = blockIdx.x, t = blockIdx.y, tid = threadIdx.x;
+  const int x0 = chunk * WG_CHUNK, x1 = min(V, x0 + WG_CHUNK);
+  double acc[WG_SLOTS];
+#pragma unroll
+  for (int k = 0; k < WG_SLOTS; k++) acc[k] = 0;
+  for (int xs = x0; xs < x1; xs += WG_SUB) {
+    const int n = min(WG_SUB, x1 - xs);
+    if (tid < n) ys[tid] = tap_site(nbr, t, xs + tid, V);
+    __syncthreads();
+    for (int e = tid; e < WG_SUB * SG_NC; e += SGB) {
+      const int x = e % WG_SUB, o = e / WG_SUB;
+      const bool ok = x < n;
+      const size_t ix = (size_t)o * V + xs + x;
+      A1[0][x][o] = ok ? in.az1d[ix] : 0.f;
+      A1[1][x][o] = ok ? in.az1[ix] : 0.f;
+      A2[0][x][o] = ok ? in.az2d[ix] : 0.f;
+      A2[1][x][o] = ok ? in.az2[ix] : 0.f;
+      const size_t iy = (size_t)o * V + (ok ? ys[x] : 0);
+      B2[0][x][o] = ok ? in.h1d[iy] : 0.f;
+      B2[1][x][o] = ok ? in.h1[iy] : 0.f;
+    }
+    for (int e = tid; e < WG_SUB * SG_NF; e += SGB) {
+      const int x = e % WG_SUB, i = e / WG_SUB;
+      const bool ok = x < n;
+      const size_t iy = (size_t)i * V + (ok ? ys[x] : 0);
+      B1[0][x][i] = ok ? in.v[iy] : 0.f;
+      B1[1][x][i] = ok ? in.phi[iy] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < WG_SLOTS; k++) {
+      const int p = tid + k * SGB;
+      if (p < WG_P1) {
+        const int o = p / SG_NF, i = p % SG_NF;
+        double s = 0;
+        for (int x = 0; x < WG_SUB; x++)
+          s += (double)A1[0][x][o] * B1[0][x][i] + (double)A1[1][x][o] * B1[1][x][i];
+        acc[k] += s;
+      } else if (p < WG_PT) {
+        const int q = p - WG_P1, o = q / SG_NC, i = q % SG_NC;
+        double s = 0;
+        for (int x = 0; x < WG_SUB; x++)
+          s += (double)A2[0][x][o] * B2[0][x][i] + (double)A2[1][x][o] * B2[1][x][i];
+        acc[k] += s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < WG_SLOTS; k++) {
+    const int p = tid + k * SGB;
+    if (p < WG_PT) part[((size_t)chunk * SG_NT + t) * WG_PT + p] = acc[k];
+  }
+  if (t == 0 && tid < WG_NB) {
+    const float *f = tid < SG_NC ? in.az1 : tid < 2 * SG_NC ? in.az2 : in.h2d;
+    const int o = tid % SG_NC;
+    double s = 0;
+    for (int x = x0; x < x1; x++) s += f[(size_t)o * V + x];
+    bpart[(size_t)chunk * WG_NB + tid] = s;
+  }
+}
+
+__global__ void __launch_bounds__(SGB)
+k_sg_wsum(const double *__restrict__ part, const double *__restrict__ bpart, int nchunk,
+          double *__restrict__ gacc, double scale) {
   const int p = blockIdx.x * SGB + threadIdx.x;
   if (p >= SG_NP) return;
   double s = 0;
-  if (p < SG_B1) {
-    const int t = p / (SG_NC * SG_NF), o = (p / SG_NF) % SG_NC, i = p % SG_NF;
-    for (int x = 0; x < V; x++) {
-      const int y = tap_site(nbr, t, x, V);
-      s += (double)in.az1d[(size_t)o * V + x] * in.v[(size_t)i * V + y] +
-           (double)in.az1[(size_t)o * V + x] * in.phi[(size_t)i * V + y];
-    }
-  } else if (p < SG_W2) {
-    const int o = p - SG_B1;
-    for (int x = 0; x < V; x++) s += in.az1[(size_t)o * V + x];
-  } else if (p < SG_B2) {
-    const int q = p - SG_W2;
-    const int t = q / (SG_NC * SG_NC), o = (q / SG_NC) % SG_NC, i = q % SG_NC;
-    for (int x = 0; x < V; x++) {
-      const int y = tap_site(nbr, t, x, V);
-      s += (double)in.az2d[(size_t)o * V + x] * in.h1d[(size_t)i * V + y] +
-           (double)in.az2[(size_t)o * V + x] * in.h1[(size_t)i * V + y];
-    }
-  } else if (p < SG_W3) {
-    const int o = p - SG_B2;
-    for (int x = 0; x < V; x++) s += in.az2[(size_t)o * V + x];
+  if (p < SG_B1 || (p >= SG_W2 && p < SG_B2)) {
+    const int t = p < SG_B1 ? p / WG_P1 : (p - SG_W2) / (SG_NC * SG_NC);
+    const int loc = p < SG_B1 ? p % WG_P1 : WG_P1 + (p - SG_W2) % (SG_NC * SG_NC);
+    for (int c = 0; c < nchunk; c++) s += part[((size_t)c * SG_NT + t) * WG_PT + loc];
   } else {
-    const int o = p - SG_W3;
-    for (int x = 0; x < V; x++) s += in.h2d[(size_t)o * V + x];
+    const int b = p < SG_W2 ? p - SG_B1 : p < SG_W3 ? SG_NC + p - SG_B2 : 2 * SG_NC + p - SG_W3;
+    for (int c = 0; c < nchunk; c++) s += bpart[(size_t)c * WG_NB + b];
   }
   gacc[p] += s * scale;
 }
@@ -633,7 +696,7 @@ static int sg_alloc_f(float **p, size_t n) {
 void sg_free(susy_ctx *c) {
   if (!c->sg) return;
   SgState &S = *c->sg;
-  void *ptrs[] = {S.d_base, S.d_diff, S.theta, S.adam_m, S.adam_v, S.wbf, S.gacc, S.phi, S.h1,
+  void *ptrs[] = {S.d_base, S.d_diff, S.theta, S.adam_m, S.adam_v, S.wbf, S.gacc, S.wpart, S.phi, S.h1,
                   S.h2, S.gz2, S.gz1, S.gphi, S.v, S.z1d, S.h1d, S.h2d, S.az2d, S.az2, S.az1d,
                   S.az1, S.F, S.R, S.Frec};
   for (void *p : ptrs)
@@ -664,6 +727,10 @@ extern "C" int susy_sg_init(susy_ctx *c, int seed, double scale) {
   SGRET(sg_alloc_f(&S.adam_v, SG_NP));
   HIPCHK(hipMalloc(&S.wbf, sizeof(unsigned short) * SG_NP));
   HIPCHK(hipMalloc(&S.gacc, sizeof(double) * SG_NP));
+  {
+    const size_t nchunk = (V + WG_CHUNK - 1) / WG_CHUNK;
+    HIPCHK(hipMalloc(&S.wpart, sizeof(double) * nchunk * (SG_NT * WG_PT + WG_NB)));
+  }
   float **planes65[] = {&S.phi, &S.gphi, &S.v};
   for (float **p : planes65) SGRET(sg_alloc_f(p, SG_NF * V));
   float **planes16[] = {&S.h1, &S.h2, &S.gz2, &S.gz1, &S.z1d, &S.h1d, &S.h2d, &S.az2d, &S.az2,
@@ -808,8 +875,11 @@ static int sg_loss_grad(susy_ctx *c, const double2 *U, const double2 *Ft, double
   hipLaunchKernelGGL(k_sg_tan3, dim3(nb), dim3(SGB), 0, c->s, S.wbf, S.az2d, S.az2, S.h1, S.z1d,
                      c->d_nbr, S.az1d, S.az1, V);
   SgGradIn in = {S.phi, S.v, S.h1, S.h1d, S.h2d, S.az2d, S.az2, S.az1d, S.az1};
-  hipLaunchKernelGGL(k_sg_wgrad, dim3((SG_NP + SGB - 1) / SGB), dim3(SGB), 0, c->s, in,
-                     c->d_nbr, S.gacc, V, gscale / sc[1]);
+  const int nchunk = (V + WG_CHUNK - 1) / WG_CHUNK;
+  hipLaunchKernelGGL(k_sg_wpart, dim3(nchunk, SG_NT), dim3(SGB), 0, c->s, in, c->d_nbr, S.wpart,
+                     S.wpart + (size_t)nchunk * SG_NT * WG_PT, V);
+  hipLaunchKernelGGL(k_sg_wsum, dim3((SG_NP + SGB - 1) / SGB), dim3(SGB), 0, c->s, S.wpart,
+                     S.wpart + (size_t)nchunk * SG_NT * WG_PT, nchunk, S.gacc, gscale / sc[1]);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -53,10 +53,13 @@ def test_features_action_force(ncol):
     assert rel(F, F_ref) < 1e-5
 
 
-@pytest.mark.parametrize("ncol", [2, 4])
-def test_force_matching_gradient(ncol):
-    say("surrogate grad", ncol)
-    lat, U, th, geo = setup(ncol)
+@pytest.mark.parametrize("ncol,dims", [(2, (2, 4, 2, 4)), (4, (2, 4, 2, 4)),
+                                       (2, (6, 6, 6, 6))])
+def test_force_matching_gradient(ncol, dims):
+    """(6^4: 1296 sites, two site chunks of the weight-gradient reduction,
+    the last one ragged)"""
+    say("surrogate grad", ncol, dims)
+    lat, U, th, geo = setup(ncol, dims=dims)
     Ft = random_links(lat.link_shape(), 9, 0.5) - np.eye(ncol)
     loss, g = lat.sg_loss_grad(U, Ft)
     loss_ref, g_ref = sr.force_loss_grad(geo, th, U, Ft)
