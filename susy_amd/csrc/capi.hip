@@ -178,8 +178,10 @@ static int ctx_setup(susy_ctx *c) {
   RET(dalloc(&c->d_tmp, tfn));
   RET(dalloc(&c->d_boundary, tfn));
   const int tsl = p.nx * p.ny * p.nz;
-  int ds_blocks = std::max({ds_grid(V, tsl, 0, nullptr), ds_grid(V, tsl, 1, nullptr),
-                            ds_grid(V, tsl, 2, nullptr)});
+  int ds_blocks = 0;
+  for (int order = 0; order <= 2; order++)
+    for (int sites : {DS_SITES64, DS_SITES32})
+      ds_blocks = std::max(ds_blocks, ds_grid(V, tsl, order, sites, nullptr));
   c->npart = std::max({ds_blocks, 2048, 4 * ((V + 255) / 256), (NUMLINK * V + 255) / 256}) + 64;
   RET(dalloc(&c->d_part, (size_t)c->npart * 2));
   RET(dalloc(&c->d_scal, (size_t)64));

@@ -174,7 +174,9 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
   for (int a = 0; a <= NUMLINK; a++) st_keep(aux + (size_t)a * V + s, t[a]);
 }
 
-#define DS_THREADS (DS_SITES * DS_SPLIT)
+template <typename R> struct DsSites;
+template <> struct DsSites<double> { static constexpr int n = DS_SITES64; };
+template <> struct DsSites<float> { static constexpr int n = DS_SITES32; };
 
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
@@ -189,7 +191,8 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
 // first stores.  Fewer sites are then in flight per XCD at the same
 // occupancy, so their neighbourhoods compete less for the XCD's L2.
 template <typename R, bool DAG, int EPI>
-__global__ void __launch_bounds__(DS_THREADS) __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
+__global__ void __launch_bounds__(DsSites<R>::n * DS_SPLIT)
+    __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
@@ -203,6 +206,7 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   // (the flag is one word read by every thread: the exit is block-uniform)
   if (skip != nullptr && *skip) return;
   const int L = blockIdx.x;
+  constexpr int DS_SITES = DsSites<R>::n, DS_THREADS = DS_SITES * DS_SPLIT;
   const int lane = threadIdx.x % DS_SITES, w = threadIdx.x / DS_SITES;
   const int q = L >> 3;                       // position in this XCD's queue
   const int comp = q % NTF;
@@ -499,8 +503,10 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     HIPCHK(hipGetLastError());
   }
   DsOrder ord;
+  constexpr int threads = DsSites<R>::n * DS_SPLIT;
   const int nb = ds_grid(V, c->p.nx * c->p.ny * c->p.nz,
-                         sizeof(R) == sizeof(float) ? c->site_order32 : c->site_order, &ord);
+                         sizeof(R) == sizeof(float) ? c->site_order32 : c->site_order,
+                         DsSites<R>::n, &ord);
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, aux_out, c->d_nbr, c->d_t, V, c->p.nt, pbc, \
@@ -517,14 +523,14 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   const unsigned lds = sizeof(R) == sizeof(double) ? DS_OCC_LDS64 : DS_OCC_LDS32;
   if (p0 == nullptr) {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 0>), dim3(nb), dim3(threads), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 0>), dim3(nb), dim3(threads), lds, c->s, DS_ARGS);
   } else {
     if (sign == 1)
-      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, false, 1>), dim3(nb), dim3(threads), lds, c->s, DS_ARGS);
     else
-      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(DS_THREADS), lds, c->s, DS_ARGS);
+      hipLaunchKernelGGL((k_dslash<R, true, 1>), dim3(nb), dim3(threads), lds, c->s, DS_ARGS);
   }
 #undef DS_ARGS
   HIPCHK(hipGetLastError());

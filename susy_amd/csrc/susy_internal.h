@@ -25,10 +25,14 @@
 #define NTERMS 30
 #define NQLINK 28
 #define IMAG_TOL 1.0e-8
-// operator kernel: a block covers DS_SITES sites of one output component,
-// with DS_SPLIT wavefronts sharing the products of each output matrix
-#ifndef DS_SITES
-#define DS_SITES 128
+// operator kernel: a block covers DS_SITES64 (FP64) / DS_SITES32 (FP32)
+// sites of one output component, with DS_SPLIT wavefronts sharing the
+// products of each output matrix (measured best: 256 / 128 / 1)
+#ifndef DS_SITES64
+#define DS_SITES64 256
+#endif
+#ifndef DS_SITES32
+#define DS_SITES32 128
 #endif
 #ifndef DS_SPLIT
 #define DS_SPLIT 1
@@ -51,7 +55,7 @@ struct DsOrder {
 };
 // grid size (blocks) of the operator kernel, and its order parameters;
 // tslice = sites per t-slice (nx ny nz)
-inline int ds_grid(int V, int tslice, int paired, DsOrder *o) {
+inline int ds_grid(int V, int tslice, int paired, int DS_SITES, DsOrder *o) {
   DsOrder d;
   d.paired = paired;
   d.half = V / 2;
