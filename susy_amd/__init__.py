@@ -405,7 +405,9 @@ class Lattice:
         return it.value
 
 
-def set_cg_options(ncol, window=32, reliable=128):
+def set_cg_options(ncol, window=0, reliable=128):
+    """multi-shift CG window (0: the default, 64) and FP32-form
+    reliable-update period"""
     r = load(ncol).susy_set_cg_options(window, reliable)
     if r < 0:
         raise RuntimeError(load(ncol).susy_last_error().decode())
