@@ -118,8 +118,8 @@ int susy_ncol(void);
 const char *susy_last_error(void);
 // Host-only consistency check of the operator/force term tables (0 = ok).
 int susy_validate_tables(void);
-// CG tuning: residual window of the shifted-system updates (1, 4, 8, 16, 32
-// or 64 iterations; 0 = the default, 64) and, for
+// CG tuning: residual window of the shifted-system updates (1, 4, 8, 16 or
+// 32 iterations; 0 = the default, 32) and, for
 // the FP32 form, iterations between FP64 true residuals (default 128).
 // Process-wide.
 int susy_set_cg_options(int window, int reliable);

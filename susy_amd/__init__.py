@@ -406,7 +406,7 @@ class Lattice:
 
 
 def set_cg_options(ncol, window=0, reliable=128):
-    """multi-shift CG window (0: the default, 64) and FP32-form
+    """multi-shift CG window (0: the default, 32) and FP32-form
     reliable-update period"""
     r = load(ncol).susy_set_cg_options(window, reliable)
     if r < 0:

@@ -37,7 +37,7 @@
 
 #define CB 256
 #define MAXSH 16
-#define MAXWIN 64
+#define MAXWIN 32
 
 // ---------------------------------------------------------------------
 template <typename C>
@@ -385,7 +385,7 @@ k_cg_p0_prep(const C *pin, const C *__restrict__ r, C *pout, const C *__restrict
 struct PPtrs {
   void *p[MAXSH];
 };
-template <typename C>
+template <typename C, int M>
 __global__ void __launch_bounds__(CB)
 k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__restrict__ st,
            const int *__restrict__ skip, size_t n) {
@@ -393,51 +393,32 @@ k_cg_flush(PPtrs pp, PsiPtrs psi, const C *__restrict__ win, const CgState *__re
   if ((skip != nullptr && *skip) || cnt == 0) return;
   size_t i = (size_t)blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
+  // only this window's slots are read (the others may hold data of another
+  // solve, in either precision)
+  double2 w[M];
+#pragma unroll
+  for (int l = 0; l < M; l++) w[l] = l < cnt ? to64(win[(size_t)l * n + i]) : make_double2(0, 0);
   const int nsh = st->nsh;
-  // accumulators of every shift in registers; the window's live slots are
-  // read once each, in order (the others may hold data of another solve,
-  // in either precision); per shift the terms add in the order A p, E p,
-  // then slot by slot
-  double2 np[MAXSH], x[MAXSH];
+  for (int j = 0; j < nsh; j++) {
+    if (!st->touched[j]) continue;
+    C *pk = (C *)pp.p[j] + i;
+    double2 p = to64(*pk);
+    double2 x = psi.p[j][i];
+    const double A = st->A[j], E = st->E[j];
+    vm_drain();   // this shift's loads complete before its stores
+    double2 np = make_double2(A * p.x, A * p.y);
+    x.x += E * p.x;
+    x.y += E * p.y;
 #pragma unroll
-  for (int j = 0; j < MAXSH; j++) {
-    np[j] = x[j] = make_double2(0, 0);
-    if (j < nsh && st->touched[j]) {
-      const double2 p = to64(((const C *)pp.p[j])[i]);
-      const double A = st->A[j], E = st->E[j];
-      np[j] = make_double2(A * p.x, A * p.y);
-      x[j] = psi.p[j][i];
-      x[j].x += E * p.x;
-      x[j].y += E * p.y;
+    for (int l = 0; l < M; l++) {
+      const double cl = st->Cw[j][l], dl = st->Dw[j][l];
+      np.x += cl * w[l].x;
+      np.y += cl * w[l].y;
+      x.x += dl * w[l].x;
+      x.y += dl * w[l].y;
     }
-  }
-  for (int l = 0; l < cnt; l++) {
-    const double2 w = to64(win[(size_t)l * n + i]);
-#pragma unroll
-    for (int j = 0; j < MAXSH; j++) {
-      if (j < nsh && st->touched[j]) {
-        const double cl = st->Cw[j][l], dl = st->Dw[j][l];
-        np[j].x += cl * w.x;
-        np[j].y += cl * w.y;
-        x[j].x += dl * w.x;
-        x[j].y += dl * w.y;
-      }
-    }
-  }
-  vm_drain();   // every load before the first store
-  C pv[MAXSH];
-#pragma unroll
-  for (int j = 0; j < MAXSH; j++) {
-    pv[j] = from64<C>(np[j]);
-    if (j < nsh && st->touched[j]) {
-      if (j > 0) ((C *)pp.p[j])[i] = pv[j];
-      psi.p[j][i] = x[j];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < MAXSH; j++) {
-    hold(pv[j]);
-    hold(x[j]);
+    if (j > 0) st_keep(pk, from64<C>(np));
+    st_keep(psi.p[j] + i, x);
   }
 }
 
@@ -465,24 +446,21 @@ static int nblk(size_t n) {
   return (int)(nb < 2048 ? nb : 2048);
 }
 
-// residuals kept per window (<= MAXWIN), per precision: the flush costs one
-// read of each residual plus a read and write of every active p_j and psi_j,
-// so a longer window amortises the second part (bench trajectory, window 64
-// against 32: FP32 -0.7%, FP64 -1.6%)
-static int g_window64 = 64, g_window32 = 64;
+// residuals kept per window (<= MAXWIN), per precision
+static int g_window64 = 32, g_window32 = 32;
 // FP32 form: iterations between true residuals; a multiple of the window, so
 // the reliable update's flush finds the window already flushed
 static int g_reliable = 128;
 
 extern "C" int susy_set_cg_options(int window, int reliable) {
   if ((window != 0 && window != 1 && window != 4 && window != 8 && window != 16 &&
-       window != 32 && window != 64) ||
+       window != 32) ||
       reliable < 1) {
-    set_error("cg options: window in {0,1,4,8,16,32,64}, reliable >= 1");
+    set_error("cg options: window in {0,1,4,8,16,32}, reliable >= 1");
     return SUSY_E_ARG;
   }
-  g_window64 = window ? window : 64;
-  g_window32 = window ? window : 64;
+  g_window64 = window ? window : 32;
+  g_window32 = window ? window : 32;
   g_reliable = reliable;
   return 0;
 }
@@ -569,7 +547,13 @@ static int cg_impl(susy_ctx *c, const double2 *src, double2 *const *psim, int ns
     pp.p[0] = pfrozen;
     for (int j = 1; j < nsh; j++) pp.p[j] = pj + (size_t)(j - 1) * n;
     dim3 g((n + CB - 1) / CB), b(CB);
-    hipLaunchKernelGGL((k_cg_flush<C>), g, b, 0, c->s, pp, psi, win, st, skip, n);
+    switch (m) {
+      case 1: hipLaunchKernelGGL((k_cg_flush<C, 1>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 4: hipLaunchKernelGGL((k_cg_flush<C, 4>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 8: hipLaunchKernelGGL((k_cg_flush<C, 8>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      case 16: hipLaunchKernelGGL((k_cg_flush<C, 16>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+      default: hipLaunchKernelGGL((k_cg_flush<C, 32>), g, b, 0, c->s, pp, psi, win, st, skip, n); break;
+    }
     hipLaunchKernelGGL(k_cgs_reset, dim3(1), dim3(STATE_THREADS), 0, c->s, st, skip);
     HIPCHK(hipGetLastError());
     host_cnt = 0;
