@@ -249,6 +249,18 @@ int susy_plaq_moments_dev(susy_ctx *c, double *det5, double *w3);
 int susy_eig_dsq(susy_ctx *c, int nlow, int nhigh, int maxiter, double tol, int seed,
                  double *low, double *high, int *iters);
 
+/* Pfaffian of the fermion operator (phase.c, printed as PFAFF in
+   testsuite/scalar/phase.U*.ref): M_ab = <e_a|D|e_b> (bilinear) in the
+   reference's generator basis, built column by column with the FP64
+   operator; log|Pf M| and arg Pf M in [0, 2 pi); asym (may be NULL) =
+   max|M + M^T| / max|M|.  For lattices with 16 N^2 V <= 8192 (2^4). */
+int susy_pfaffian_dev(susy_ctx *c, double *log_mag, double *phase, double *asym);
+
+/* The same matrix M (phase.c:36-213, matvec): n = 16 N^2 V, basis entry
+   (site, generator j, slot q) at index (site * N^2 + j) * 16 + q, written
+   row-major M[a][b] with (re, im) interleaved into 2 n^2 doubles. */
+int susy_fermion_matrix_dev(susy_ctx *c, double *m);
+
 /* Learned surrogate action S_theta(U) (no counterpart in the reference; its
    judge is the plain fermion_force).  Two gauge-invariant convolution layers
    (16 channels, bfloat16 weights, FP32 accumulation) over plaquette and

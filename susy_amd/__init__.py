@@ -54,7 +54,7 @@ EXPORTS = [
     "susy_sg_eval", "susy_sg_features", "susy_sg_loss_grad", "susy_sg_record", "susy_sg_pairs",
     "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
-    "susy_plaq_moments_dev",
+    "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
 ]
 
 _libs = {}
@@ -118,6 +118,8 @@ def load(ncol):
         "susy_wilson_line_dev": (i, [vp, i, i, dp, dp, dp]),
         "susy_scalar_eig_dev": (i, [vp, i, dp, dp, dp, dp]),
         "susy_plaq_moments_dev": (i, [vp, dp, dp]),
+        "susy_pfaffian_dev": (i, [vp, dp, dp, dp]),
+        "susy_fermion_matrix_dev": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -329,6 +331,21 @@ class Lattice:
         d5, w3 = np.zeros(5), np.zeros(3)
         self._check(self.L.susy_plaq_moments_dev(self.h, _dp(d5), _dp(w3)))
         return d5, w3
+
+    def pfaffian(self):
+        """(log |Pf M|, arg Pf M in [0, 2 pi), max|M + M^T| / max|M|) of the
+        fermion operator in the reference's phase-measurement basis"""
+        lm, ph, asym = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.L.susy_pfaffian_dev(self.h, ctypes.byref(lm), ctypes.byref(ph),
+                                             ctypes.byref(asym)))
+        return lm.value, ph.value, asym.value
+
+    def fermion_matrix(self):
+        """n x n complex matrix of the fermion operator in the same basis"""
+        n = 16 * self.ncol ** 2 * self.V
+        m = np.zeros((n, n), dtype=np.complex128)
+        self._check(self.L.susy_fermion_matrix_dev(self.h, m.ctypes.data))
+        return m
 
     def eig_dsq(self, nlow, nhigh, maxiter=600, tol=1e-10, seed=1):
         """(nlow smallest, nhigh largest eigenvalues of DSq, Lanczos iterations)"""

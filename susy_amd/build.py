@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 BUILDDIR = os.path.join(HERE, "_build")
 SOURCES = ["lattice.hip", "dslash.hip", "fields.hip", "force.hip", "blas.hip",
-           "cg.hip", "capi.hip", "surrogate.hip", "eig.hip", "meas.hip"]
+           "cg.hip", "capi.hip", "surrogate.hip", "eig.hip", "meas.hip", "pfaff.hip"]
 NCOLS = (2, 3, 4)
 ARCH = os.environ.get("SUSY_HIP_ARCH", "gfx950")
 

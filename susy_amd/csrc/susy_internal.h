@@ -251,6 +251,7 @@ struct susy_ctx {
   // index 0 FP64, 1 FP32): every prof_every-th launch is bracketed by events
   int site_order = 2;          // DsOrder::paired of the operator kernel (FP64)
   int site_order32 = 2;        // the same for the FP32 operator
+  int pf_handle = -1;          // two Twist_Fermion handles of the Pfaffian (susy_pfaffian_dev)
   SgState *sg = nullptr;       // learned surrogate state (susy_sg_init)
   int prof_every = 0;
   long prof_launch[2] = {0, 0};
