@@ -128,6 +128,9 @@ int susy_set_verbose(int v);
 int susy_create(const susy_params *p, susy_ctx **out);
 void susy_destroy(susy_ctx *c);
 int susy_volume(const susy_ctx *c);
+// The context's RHMC coefficients (setup_rhmc.c:882-1031, the Nroot row):
+// out[62] = ampdeg4, amp4[15], shift4[15], ampdeg8, amp8[15], shift8[15].
+int susy_rhmc_coeffs(const susy_ctx *c, double *out);
 
 // Site order is the reference's (even sites first, layout_hyper_prime.c:157).
 // Gather table g in 0..73 as numbered by the reference (com_vanilla.c,

@@ -41,7 +41,7 @@ EXPORTS = [
     "fermion_op", "DSq", "congrad_multi", "fermion_force", "gauge_force",
     "susy_bind_lattice", "susy_bind_sites", "susy_bind_couplings", "susy_bind_rhmc",
     "susy_links_changed", "susy_ncol", "susy_last_error", "susy_create", "susy_destroy",
-    "susy_volume", "susy_get_gather", "susy_set_links", "susy_get_links", "susy_set_mom",
+    "susy_volume", "susy_rhmc_coeffs", "susy_get_gather", "susy_set_links", "susy_get_links", "susy_set_mom",
     "susy_get_mom", "susy_load_config", "susy_cold_start", "susy_get_fields",
     "susy_tf_alloc", "susy_tf_upload", "susy_tf_download", "susy_fermion_op_dev",
     "susy_dsq_dev", "susy_congrad_multi_dev", "susy_fermion_force_dev",
@@ -78,7 +78,8 @@ def load(ncol):
     sig = {
         "susy_ncol": (i, []), "susy_validate_tables": (i, []), "susy_last_error": (ctypes.c_char_p, []),
         "susy_create": (i, [P(SusyParams), P(vp)]), "susy_destroy": (None, [vp]),
-        "susy_volume": (i, [vp]), "susy_get_gather": (i, [vp, i, ip]),
+        "susy_volume": (i, [vp]), "susy_rhmc_coeffs": (i, [vp, dp]),
+        "susy_get_gather": (i, [vp, i, ip]),
         "susy_set_links": (i, [vp, dp]), "susy_get_links": (i, [vp, dp]),
         "susy_set_mom": (i, [vp, dp]), "susy_get_mom": (i, [vp, dp]),
         "susy_load_config": (i, [vp, ctypes.c_char_p]), "susy_cold_start": (i, [vp]),
@@ -194,6 +195,12 @@ class Lattice:
     def set_mom(self, mom):
         a = np.ascontiguousarray(mom, dtype=np.complex128)
         self._check(self.L.susy_set_mom(self.h, _dp(a)))
+
+    def rhmc(self):
+        """(62,) ampdeg4, amp4[15], shift4[15], ampdeg8, amp8[15], shift8[15]"""
+        out = np.zeros(62)
+        self._check(self.L.susy_rhmc_coeffs(self.h, _dp(out)))
+        return out
 
     def get_mom(self):
         a = np.zeros(self.link_shape(), dtype=np.complex128)

@@ -23,6 +23,23 @@
 extern "C" int susy_ncol(void) { return NCOL; }
 extern "C" int susy_volume(const susy_ctx *c) { return c ? c->V : SUSY_E_STATE; }
 
+// The context's rational-approximation coefficients (the Nroot row of the
+// tables of setup_rhmc.c:882-1031): out[0] ampdeg4, out[1..15] amp4,
+// out[16..30] shift4, out[31] ampdeg8, out[32..46] amp8, out[47..61] shift8.
+extern "C" int susy_rhmc_coeffs(const susy_ctx *c, double *out) {
+  if (!c) return SUSY_E_STATE;
+  if (!out) return SUSY_E_ARG;
+  out[0] = c->ampdeg4;
+  out[31] = c->ampdeg8;
+  for (int j = 0; j < SUSY_RHMC15_NORDER; j++) {
+    out[1 + j] = c->amp4[j];
+    out[16 + j] = c->shift4[j];
+    out[32 + j] = c->amp8[j];
+    out[47 + j] = c->shift8[j];
+  }
+  return 0;
+}
+
 static bool g_tables_ready = false;
 
 // Every gather id used by the fused kernels must name one of the 74 tables.

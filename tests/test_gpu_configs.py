@@ -122,6 +122,7 @@ def test_config2_mixed_precision_errors():
     lat.update(susy_amd.FORM_FP64)
     U0 = lat.get_links()
     rh = Oracle(4, dims=(2, 2, 2, 2), Nroot=2).rhmc()
+    assert np.array_equal(lat.rhmc(), rh)
     amps, shifts = rh[1:16], rh[16:31]
     h = lat.tf_alloc(1 + 15 + 15 + 15)
     lat.reseed(11)
