@@ -19,7 +19,9 @@
 //     vs oracle/_ref on the same inputs (U(2)), adjointness <a,Db> = <D^+a,b>,
 //     free-field spectrum, gauge covariance, and end-to-end vs the
 //     reference's own regression outputs testsuite/scalar/hmc.U{2,3,4}.ref
-//     (actions to 8 digits, exact CG iteration counts).
+//     (actions to 8 digits, exact CG iteration counts), and at U(5), U(7)
+//     hmc_meas.U{5,7}.ref (start S to 12 digits; iteration counts within
+//     0.1%, where LAPACK's LU order is not restated).
 //   * determinant / inverse: U U^-1 = 1 and det against numpy; the LAPACK
 //     pivoting convention is restated and pinned end to end by the det
 //     action of hmc.U*.ref (kappa_u1 = 1 there).
@@ -36,7 +38,7 @@
 #include "../include/susy_rhmc15.h"
 
 #ifndef NCOL
-#error "compile with -DNCOL=2|3|4"
+#error "compile with -DNCOL=N (N >= 2: the gauge group U(N))"
 #endif
 #define DIMF (NCOL * NCOL)
 #define NUMLINK 5
