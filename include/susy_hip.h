@@ -29,7 +29,7 @@ extern "C" {
 #endif
 
 #ifndef NCOL
-#error "define NCOL (2, 3 or 4) to match the library before including susy_hip.h"
+#error "define NCOL (2, 3, 4 or 5) to match the library before including susy_hip.h"
 #endif
 
 // ---------------------------------------------------------------------

@@ -15,7 +15,7 @@ LIBDIR = os.path.join(HERE, "lib")
 BUILDDIR = os.path.join(HERE, "_build")
 SOURCES = ["lattice.hip", "dslash.hip", "fields.hip", "force.hip", "blas.hip",
            "cg.hip", "capi.hip", "surrogate.hip", "eig.hip", "meas.hip", "pfaff.hip"]
-NCOLS = (2, 3, 4)
+NCOLS = (2, 3, 4, 5)
 ARCH = os.environ.get("SUSY_HIP_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

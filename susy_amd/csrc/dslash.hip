@@ -34,9 +34,12 @@ int dslash_init_tables(const PlaqTerms &pt) {
   return 0;
 }
 
-// minimum wavefronts per SIMD the register allocator must allow
+// minimum wavefronts per SIMD the register allocator must allow.  From
+// U(5) on an FP64 output matrix and its operands exceed 256 registers; at one
+// wave per SIMD the allocator keeps the excess in AGPRs instead of spilling
+// it to scratch memory.
 #ifndef DS_WAVES
-#define DS_WAVES 2
+#define DS_WAVES (NCOL >= 5 ? 1 : 2)
 #endif
 // dynamic LDS bytes requested per block: caps the blocks resident per CU
 // (160 KB of LDS per CU), and with it the sites in flight per XCD, whose

@@ -30,7 +30,15 @@ __device__ __forceinline__ void mcopy(Mat d, const Mat m) {
 }
 
 // determinant by LU with partial pivoting on |re| + |im|
-__device__ __forceinline__ double2 mdet(Mat m) {
+__device__ __forceinline__ double2 mdet(const Mat in) {
+  // eliminate on a local copy: the row swaps below then stay in registers
+  // (on the caller's array they compile to pointer selects, and from U(5) on
+  // the matrix is kept in scratch memory)
+  Mat m;
+#pragma unroll
+  for (int i = 0; i < NCOL; i++)
+#pragma unroll
+    for (int j = 0; j < NCOL; j++) m[i][j] = in[i][j];
   double2 d = make_double2(1, 0);
 #pragma unroll
   for (int k = 0; k < NCOL; k++) {

@@ -42,7 +42,7 @@ def site_dtype(ncol):
                      ("f_U", mat, (5,)), ("bc", np.float64, 10)], align=True)
 
 
-@pytest.mark.parametrize("ncol", [2, 3, 4])
+@pytest.mark.parametrize("ncol", [2, 3, 4, 5])
 def test_header_compiles_and_links_as_c(ncol, tmp_path):
     src = tmp_path / "driver.c"
     src.write_text(C_DRIVER)

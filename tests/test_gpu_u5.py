@@ -1,0 +1,160 @@
+"""U(5) on the GPU: libsusy_hip_U5 against the C oracle on random links
+(gather tables, link-derived fields, fermion_op in both precisions, DSq, the
+15-shift solve, forces, update_u and the actions), then the reference's
+regression run testsuite/scalar/hmc_meas.U5.ref (1x1x1x4, fresh unit
+links): two trajectories against the printed numbers and against the
+oracle's run of the same input, both with the tolerances
+tests/test_hmc_meas_ref.py states for N > 4 (on this 4-site lattice the
+CG stopping points move by an iteration or two with the reductions'
+rounding: 3254 / 3256 / 3259 iterations on the GPU, in the oracle and in
+the reference), then the measurements printed after each trajectory (GMES,
+FLINK, LINES, LINES_POLAR, DET, WIDTHS, UUBAR_EIG, POLAR_EIG), each row
+within 2e-5 of its largest entry."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import susy_amd
+from oracle_lib import GOLDEN, Oracle, random_links, random_tf, rel_err
+from refcheck import hmc_kwargs, printed_ok
+from test_hmc_meas_ref import check_traj_loose
+
+pytestmark = pytest.mark.gpu
+
+REF = json.load(open(os.path.join(GOLDEN, "hmc_meas_ref.json")))["U5"]
+# antiperiodic: with periodic fermion BCs these random links leave D^dag D
+# nearly singular and the 15-shift solve does not converge
+KW = dict(iseed=41, kappa_u1=1.0, bmass=1.0, G=0.3, PBC=-1)
+NCOL = 5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_u5_operator_solver_forces():
+    say("U5 kernels")
+    for dims in ((1, 1, 1, 4), (2, 2, 2, 4)):
+        O = Oracle(NCOL, dims=dims, **KW)
+        lat = susy_amd.Lattice(NCOL, dims=dims, **KW)
+        for g in range(susy_amd.NGATHER):
+            assert np.array_equal(lat.gather(g), O.gather(g)), (dims, g)
+        lat.close()
+    dims = (2, 2, 2, 4)
+    O = Oracle(NCOL, dims=dims, **KW)
+    lat = susy_amd.Lattice(NCOL, dims=dims, **KW)
+    lk = random_links(O.link_shape(), 45, 0.3)
+    O.set_links(lk)
+    lat.set_links(lk)
+    pd, ui, dm, fm = lat.fields()
+    opd, oui, odm, ofm = O.fields()
+    mask = ~np.eye(5, dtype=bool)
+    assert rel_err(pd[mask], opd[mask]) < 1e-12 and rel_err(ui, oui) < 1e-12
+    assert rel_err(dm, odm) < 1e-12 and rel_err(fm, ofm) < 1e-12
+    src = random_tf(O.tf_shape(), 7)
+    h = lat.tf_alloc(2 + 15)
+    lat.tf_upload(h, src)
+    for sign in (1, -1):
+        want = O.fermion_op(src, sign)
+        lat.fermion_op(h, h + 1, sign)
+        assert rel_err(lat.tf_download(h + 1), want) < 1e-12, sign
+        lat.fermion_op(h, h + 1, sign, form=susy_amd.FORM_FP32)
+        assert rel_err(lat.tf_download(h + 1), want) < 1e-5, sign
+    lat.dsq(h, h + 1)
+    assert rel_err(lat.tf_download(h + 1), O.dsq(src)) < 1e-12
+    rh = O.rhmc()
+    assert np.array_equal(lat.rhmc(), rh)
+    shifts, amps = rh[16:31], rh[1:16]
+    it_o, sz_o, ps_o = O.congrad_multi(src, shifts, 5000, 1e-10)
+    it, sz = lat.congrad_multi(h, h + 2, shifts, 5000, 1e-10)
+    say("cg", it, it_o, "size_r", sz, sz_o)
+    assert it_o < 5000 and it == it_o
+    # 670 iterations on a harder system than the U(2)-U(4) cases: the final
+    # recurrence residual carries the reductions' rounding (10% apart here),
+    # so the solutions are judged by their true FP64 residuals against the
+    # stopping bound errormin |src|^2, computed with the oracle's DSq
+    bound = 1e-10 * np.vdot(src, src).real
+    assert sz <= bound and sz_o <= bound
+    for j in range(15):
+        psi = lat.tf_download(h + 2 + j)
+        r = src - (O.dsq(psi) + shifts[j] * psi)
+        r2 = np.vdot(r, r).real
+        say("shift", j, "true |r|^2 / bound", r2 / bound, "vs oracle", rel_err(psi, ps_o[j]))
+        assert r2 <= 4 * bound, j
+    mom = random_tf(O.link_shape(), 5)
+    O.set_mom(mom)
+    lat.set_mom(mom)
+    assert abs(lat.gauge_force(0.07) / O.gauge_force(0.07) - 1) < 1e-12
+    assert rel_err(lat.get_mom() - mom, O.get_mom() - mom) < 1e-12
+    sol = random_tf((15,) + O.tf_shape(), 8)
+    O.set_mom(mom)
+    lat.set_mom(mom)
+    n_o = O.fermion_force(0.05, sol, amps)
+    for j in range(15):
+        lat.tf_upload(h + 2 + j, sol[j])
+    assert abs(lat.fermion_force(0.05, h + 2, amps) / n_o - 1) < 1e-10
+    assert rel_err(lat.get_mom() - mom, O.get_mom() - mom) < 1e-10
+    mom = random_tf(O.link_shape(), 6) * 0.1
+    O.set_mom(mom)
+    lat.set_mom(mom)
+    O.update_u(0.03)
+    lat.update_u(0.03)
+    assert rel_err(lat.get_links(), O.get_links()) < 1e-14
+    a, ao = lat.actions(), O.actions()
+    for k in range(4):
+        assert abs(a[k] / ao[k] - 1) < 1e-12, k
+
+
+def rows_ok(got, want, sig, name):
+    """The trajectories carry solver-tolerance differences (links ~1e-7
+    relative after one trajectory), so each printed row is judged relative
+    to its largest entry: within 2e-5 of it, or within the printed digits."""
+    assert len(got) == len(want), name
+    scale = max(abs(w) for w in want)
+    dev = max(abs(g - w) for g, w in zip(got, want)) / scale
+    say("  ", name, "max deviation / row scale", dev)
+    for k, (g, w) in enumerate(zip(got, want)):
+        assert abs(g - w) <= 2e-5 * scale or printed_ok(g, w, sig, slack=2.0), (name, k, g, w)
+
+
+def test_u5_trajectories_reproduce_hmc_meas_ref():
+    say("hmc_meas U5")
+    kw = hmc_kwargs(REF["input"])
+    lat = susy_amd.Lattice(NCOL, **kw)
+    O = Oracle(NCOL, **kw)
+    ss, st = lat.plaquette()
+    assert ss == REF["ref"]["check_plaq"][0] and st == REF["ref"]["check_plaq"][1]
+    for k, tr in enumerate(REF["ref"]["trajectories"]):
+        got = lat.update(susy_amd.FORM_FP64)
+        want = O.update()
+        say("traj", k, got[:5], "oracle", want[:5], "printed", tr["iters"])
+        assert bool(got[3]) == bool(want[3]) and abs(got[4] / want[4] - 1) < 2e-3
+        for q in (0, 1) + tuple(range(11, 23)):
+            assert abs(got[q] - want[q]) <= 1e-5 * abs(want[q]) + 1e-12, (k, q)
+        assert abs(got[2] - want[2]) < 1e-3
+        check_traj_loose(got, tr, k == 0)
+        gm = tr["gmes"]
+        re, im, mod = lat.wilson_line(3)
+        ss, st = lat.plaquette()
+        sb = lat.actions()[0] / lat.V
+        rows_ok([re, im, ss, st, sb, mod], [gm[0], gm[1], gm[3], gm[4], gm[5], gm[6]], 7,
+                ("gmes", k))
+        lt, ave, width = lat.link_trace()[:3]
+        rows_ok(list(lt) + [ave, width], tr["flink"], 6, ("flink", k))
+        for name, proj in (("lines", False), ("lines_polar", True)):
+            row = []
+            for a in range(5):
+                r, i, _ = lat.wilson_line(a, proj)
+                row += [r, i]
+            rows_ok(row, tr[name], 6, (name, k))
+        det5, w3 = lat.plaq_moments()
+        rows_ok(det5, tr["det"], 6, ("det", k))
+        rows_ok(w3, tr["widths"], 6, ("widths", k))
+        for name, proj in (("uubar_eig", False), ("polar_eig", True)):
+            ave, wd, mn, mx = lat.scalar_eig(proj)
+            for i, row in enumerate(tr[name]):
+                rows_ok([ave[i], wd[i], mn[i], mx[i]], row, 6, (name, k, i))

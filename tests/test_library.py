@@ -27,7 +27,7 @@ def test_header_lists_the_drop_in_signatures():
     assert set(names) <= set(susy_amd.EXPORTS)
 
 
-@pytest.mark.parametrize("ncol", [2, 3, 4])
+@pytest.mark.parametrize("ncol", [2, 3, 4, 5])
 def test_library_exports_every_declared_symbol(ncol):
     path = susy_amd.lib_path(ncol)
     assert os.path.exists(path), "run __graft_entry__.build() first"
