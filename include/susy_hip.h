@@ -264,8 +264,8 @@ int susy_pfaffian_dev(susy_ctx *c, double *log_mag, double *phase, double *asym)
    row-major M[a][b] with (re, im) interleaved into 2 n^2 doubles. */
 int susy_fermion_matrix_dev(susy_ctx *c, double *m);
 
-/* Learned surrogate action S_theta(U) (no counterpart in the reference; its
-   judge is the plain fermion_force).  Two gauge-invariant convolution layers
+/* Learned surrogate action S_theta(U) (no counterpart in the reference; it
+   is trained on and checked against the plain fermion_force).  Two gauge-invariant convolution layers
    (16 channels, bfloat16 weights, FP32 accumulation) over plaquette and
    1x2-loop traces; F_theta = dS_theta/dUbar in the convention of
    fermion_force.  With SUSY_FORM_SURROGATE, susy_update adds F_theta to every

@@ -1,6 +1,6 @@
 // Learned surrogate action S_theta(U) for the inner level of the MD
-// integrator (no counterpart in the reference; its judge is the plain
-// fermion force of update_h.c:928-978).
+// integrator (no counterpart in the reference; it is trained on and checked
+// against the plain fermion force of update_h.c:928-978).
 //
 //   S_theta(U) = sum_x w3 . h2(x),
 //   h2 = tanh(b2 + sum_t W2[t] h1(x + d_t)),  h1 = tanh(b1 + sum_t W1[t] phi(x + d_t))

@@ -75,7 +75,7 @@ def test_u5_operator_solver_forces():
     assert it_o < 5000 and it == it_o
     # 670 iterations on a harder system than the U(2)-U(4) cases: the final
     # recurrence residual carries the reductions' rounding (10% apart here),
-    # so the solutions are judged by their true FP64 residuals against the
+    # so the solutions are checked by their true FP64 residuals against the
     # stopping bound errormin |src|^2, computed with the oracle's DSq
     bound = 1e-10 * np.vdot(src, src).real
     assert sz <= bound and sz_o <= bound
@@ -111,7 +111,7 @@ def test_u5_operator_solver_forces():
 
 def rows_ok(got, want, sig, name):
     """The trajectories carry solver-tolerance differences (links ~1e-7
-    relative after one trajectory), so each printed row is judged relative
+    relative after one trajectory), so each printed row is compared relative
     to its largest entry: within 2e-5 of it, or within the printed digits."""
     assert len(got) == len(want), name
     scale = max(abs(w) for w in want)
