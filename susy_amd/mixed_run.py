@@ -17,7 +17,8 @@ the FP32 form's solutions and force against the FP64 form's, both forms' dH,
 plaquettes and iteration counts, and seconds per trajectory.
 
     python -m susy_amd.mixed_run [--therm 20] [--therm-form fp64] [--reliable 50]
-        [--save PATH] [--load PATH]
+        [--save PATH] [--load PATH] [--dims 12,12,12,24] [--traj-length 1]
+        [--nstep 20]
 """
 import argparse
 import json
@@ -48,10 +49,15 @@ def main(argv=None):
     ap.add_argument("--reliable", type=int, default=50)
     ap.add_argument("--save", default=None, help="write the thermalised links (MILC file)")
     ap.add_argument("--load", default=None, help="start from this MILC file, no thermalisation")
+    ap.add_argument("--dims", default="12,12,12,24", help="lattice (nx,ny,nz,nt)")
+    ap.add_argument("--traj-length", type=float, default=1.0)
+    ap.add_argument("--nstep", type=int, default=20)
     a = ap.parse_args(argv)
 
+    params = dict(PARAMS, dims=tuple(int(x) for x in a.dims.split(",")),
+                  traj_length=a.traj_length, nstep=a.nstep)
     susy_amd.set_cg_options(4, reliable=a.reliable)
-    lat = susy_amd.Lattice(4, **PARAMS)
+    lat = susy_amd.Lattice(4, **params)
     therm = []
     if a.load:
         lat.load_config(a.load)
@@ -69,14 +75,14 @@ def main(argv=None):
     U0 = lat.get_links()
     rh = lat.rhmc()
     amps, shifts = rh[1:16], rh[16:31]
-    errormin = PARAMS["error_per_site"] ** 2
+    errormin = params["error_per_site"] ** 2
     h = lat.tf_alloc(1 + 15 + 15)
     lat.reseed(11)
     lat.grsource(h, h + 1, susy_amd.FORM_FP64)
     solve = {}
     for k, (name, form) in enumerate(FORMS.items()):
         t0 = time.perf_counter()
-        it, size_r = lat.congrad_multi(h, h + 1 + 15 * k, shifts, PARAMS["niter"], errormin,
+        it, size_r = lat.congrad_multi(h, h + 1 + 15 * k, shifts, params["niter"], errormin,
                                        form=form)
         solve[name] = {"iters": it, "size_r": size_r, "s": time.perf_counter() - t0}
         say(f"solve {name} iters {it} size_r {size_r:.3e} {solve[name]['s']:.2f} s")
@@ -101,8 +107,9 @@ def main(argv=None):
         say(f"trajectory {name} dH {st[2]:.6f} accept {int(st[3])} iters {int(st[4])} "
             f"{traj[name]['s']:.1f} s")
     d64, d32 = traj["fp64"]["dH"], traj["fp32"]["dH"]
-    res = {"workload": "mixed precision U(4) 12^3x24, lambda 1, bmass 0.5, G 0.05, Nroot 2, "
-                       "traj length 1, nstep 20",
+    susy_amd.set_cg_options(4)
+    res = {"workload": f"mixed precision U(4) {a.dims}, lambda 1, bmass 0.5, G 0.05, Nroot 2, "
+                       f"traj length {a.traj_length}, nstep {a.nstep}",
            "thermalisation": {"trajectories": len(therm), "form": a.therm_form,
                               "acceptance": float(np.mean([t["accept"] for t in therm]))
                               if therm else None,
