@@ -17,8 +17,9 @@ the FP32 form's solutions and force against the FP64 form's, both forms' dH,
 plaquettes and iteration counts, and seconds per trajectory.
 
     python -m susy_amd.mixed_run [--therm 20] [--therm-form fp64] [--reliable 50]
-        [--save PATH] [--load PATH] [--dims 12,12,12,24] [--traj-length 1]
-        [--nstep 20]
+        [--save PATH] [--load PATH] [--therm-only] [--dims 12,12,12,24]
+        [--traj-length 1] [--nstep 20]
+(--load PATH --therm K continues a thermalisation saved with --save.)
 """
 import argparse
 import json
@@ -48,7 +49,8 @@ def main(argv=None):
     ap.add_argument("--therm-form", choices=sorted(FORMS), default="fp64")
     ap.add_argument("--reliable", type=int, default=50)
     ap.add_argument("--save", default=None, help="write the thermalised links (MILC file)")
-    ap.add_argument("--load", default=None, help="start from this MILC file, no thermalisation")
+    ap.add_argument("--load", default=None, help="start from this MILC file (then --therm more)")
+    ap.add_argument("--therm-only", action="store_true", help="stop after thermalising (and --save)")
     ap.add_argument("--dims", default="12,12,12,24", help="lattice (nx,ny,nz,nt)")
     ap.add_argument("--traj-length", type=float, default=1.0)
     ap.add_argument("--nstep", type=int, default=20)
@@ -61,17 +63,19 @@ def main(argv=None):
     therm = []
     if a.load:
         lat.load_config(a.load)
-        say("loaded", a.load)
-    else:
-        for k in range(a.therm):
-            t0 = time.perf_counter()
-            st = lat.update(FORMS[a.therm_form])
-            therm.append({"dH": float(st[2]), "accept": bool(st[3]), "iters": int(st[4]),
-                          "plaq": lat.plaquette()[0], "s": time.perf_counter() - t0})
-            say(f"therm {k} dH {st[2]:.4f} accept {int(st[3])} iters {int(st[4])} "
-                f"plaq {therm[-1]['plaq']:.6f} {therm[-1]['s']:.1f} s")
-        if a.save:
-            lat.save_config(a.save)
+        say("loaded", a.load, "plaquette", lat.plaquette()[0])
+    for k in range(a.therm):
+        t0 = time.perf_counter()
+        st = lat.update(FORMS[a.therm_form])
+        therm.append({"dH": float(st[2]), "accept": bool(st[3]), "iters": int(st[4]),
+                      "plaq": lat.plaquette()[0], "s": time.perf_counter() - t0})
+        say(f"therm {k} dH {st[2]:.4f} accept {int(st[3])} iters {int(st[4])} "
+            f"plaq {therm[-1]['plaq']:.6f} {therm[-1]['s']:.1f} s")
+    if a.save:
+        lat.save_config(a.save)
+    if a.therm_only:
+        susy_amd.set_cg_options(4)
+        return
     U0 = lat.get_links()
     rh = lat.rhmc()
     amps, shifts = rh[1:16], rh[16:31]
