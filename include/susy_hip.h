@@ -112,6 +112,9 @@ enum {
   SUSY_FORM_FP64 = 0,            // plain FP64 everywhere (parity form)
   SUSY_FORM_FP32 = 1,            // FP32 DSq + CG with FP64 reliable updates
   SUSY_FORM_SURROGATE = 4,       // flag: learned surrogate force on the inner MD level
+  SUSY_FORM_PUREGAUGE = 8,       // flag: fermions off, the reference's -DPUREGAUGE build
+                                 // (update_o.c:74-177, action.c:314): no pseudofermions,
+                                 // solves or fermion forces; the action has no fermion part
 };
 
 int susy_ncol(void);
@@ -192,6 +195,19 @@ int susy_grsource_dev(susy_ctx *c, int h_src, int h_psim, int form, int *iters);
 // avg/max, fermion-force avg/max for roots 0 and 1, start action parts
 // (gauge, bmass, det, fermion, mom, total), end action parts.
 int susy_update(susy_ctx *c, int form, double *stats);
+// The same trajectory in pieces, for callers that time or interleave its
+// outer MD steps; susy_update = begin, nstep x step, end.
+//   susy_traj_begin: momenta, pseudofermions and their solves, start action,
+//     saved links/momenta and the first fermion kick (update_o.c:127-208,
+//     74-82).  Abandons an unfinished trajectory.
+//   susy_traj_step: one outer step of update_step (update_o.c:80-113):
+//     gauge level, solves + kick, gauge level, solves + kick.  Returns the
+//     steps done (1..nstep).
+//   susy_traj_end: after exactly nstep steps, end action and Metropolis test
+//     (update_o.c:209-282); returns 1 accepted / 0 rejected, stats as above.
+int susy_traj_begin(susy_ctx *c, int form);
+int susy_traj_step(susy_ctx *c);
+int susy_traj_end(susy_ctx *c, double *stats);
 
 // Timing of the last call's kernels is available through the profiler; the
 // benchmark measures operator launches with these helpers (HIP events on the

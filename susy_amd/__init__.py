@@ -17,6 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 FORM_FP64 = 0
 FORM_FP32 = 1
 FORM_SURROGATE = 4      # flag: learned surrogate force on the inner MD level
+FORM_PUREGAUGE = 8      # flag: fermions off (the reference's -DPUREGAUGE build)
 # operator-kernel block order a new context starts with (susy_set_site_order:
 # FP64 order in the low byte, FP32 order in the next)
 DEFAULT_SITE_ORDER = 2 + (2 << 8)
@@ -55,6 +56,7 @@ EXPORTS = [
     "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
     "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
+    "susy_traj_begin", "susy_traj_step", "susy_traj_end",
 ]
 
 _libs = {}
@@ -92,6 +94,8 @@ def load(ncol):
         "susy_gauge_force_dev": (i, [vp, d, dp]), "susy_update_u_dev": (i, [vp, d]),
         "susy_actions_dev": (i, [vp, dp]), "susy_fermion_action_dev": (i, [vp, i, i, dp]),
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
+        "susy_traj_begin": (i, [vp, i]), "susy_traj_step": (i, [vp]),
+        "susy_traj_end": (i, [vp, dp]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
         "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
         "susy_set_site_order": (i, [vp, i]),
@@ -207,8 +211,13 @@ class Lattice:
         self._check(self.L.susy_get_mom(self.h, _dp(a)))
         return a
 
-    def load_config(self, path):
-        return self._check(self.L.susy_load_config(self.h, path.encode()))
+    def load_config(self, path, allow_bad_checksum=False):
+        """read a MILC gauge file; a checksum mismatch raises unless allowed
+        (then the return value is 1)"""
+        r = self._check(self.L.susy_load_config(self.h, os.fsencode(path)))
+        if r != 0 and not allow_bad_checksum:
+            raise RuntimeError(f"{path}: gauge-file checksums do not match")
+        return r
 
     def fields(self):
         V, n = self.V, self.ncol
@@ -282,6 +291,20 @@ class Lattice:
         """One RHMC trajectory; returns the stats vector (see susy_update)."""
         st = np.zeros(32)
         self._check(self.L.susy_update(self.h, form, _dp(st)))
+        return st
+
+    def traj_begin(self, form=FORM_FP64):
+        """start a trajectory (heatbath, start action, first fermion kick)"""
+        self._check(self.L.susy_traj_begin(self.h, form))
+
+    def traj_step(self):
+        """one outer MD step of the open trajectory; returns the steps done"""
+        return self._check(self.L.susy_traj_step(self.h))
+
+    def traj_end(self):
+        """Metropolis test of the open trajectory; returns the stats vector"""
+        st = np.zeros(32)
+        self._check(self.L.susy_traj_end(self.h, _dp(st)))
         return st
 
     def time_fermion_op(self, hs, hd, form=FORM_FP64, reps=20):

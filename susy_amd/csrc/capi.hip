@@ -108,6 +108,7 @@ extern "C" int susy_validate_tables(void) {
 
 static void free_ctx(susy_ctx *c) {
   sg_free(c);
+  traj_free(c);
   for (int p = 0; p < 2; p++)
     for (hipEvent_t e : c->prof_ev[p]) (void)hipEventDestroy(e);
   void *ptrs[] = {c->d_nbr, c->d_t, c->d_links, c->d_mom, c->d_old_links, c->d_old_mom,
@@ -751,14 +752,25 @@ extern "C" int susy_grsource_dev(susy_ctx *c, int hsrc, int hpsim, int form, int
 
 // ---------------------------------------------------------------------
 // One trajectory (update_o.c:20-282)
+// State of the trajectory in progress between susy_traj_begin, the outer
+// steps and susy_traj_end.
 struct Traj {
   susy_ctx *c;
   int form;             // precision of the CG (SUSY_FORM_FP64 / SUSY_FORM_FP32)
   bool sg = false;      // learned surrogate on the inner level
+  bool pure = false;    // fermions switched off (-DPUREGAUGE)
   std::vector<double2 *> gsrc;
   std::vector<std::vector<double2 *>> psim;
   double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
+  int iters = 0;        // CG iterations so far
+  int step = 0;         // outer steps done
+  bool open = false;    // between begin and end
+  double p0[6] = {0, 0, 0, 0, 0, 0};   // start action parts
 };
+void traj_free(susy_ctx *c) {
+  delete c->traj;
+  c->traj = nullptr;
+}
 
 static const double OM_LAMBDA = 0.193, OM_TWO_LAMBDA = 0.386, OM_LAMBDA_MID = 0.614;
 
@@ -836,6 +848,7 @@ static int traj_cg(Traj &T, int root, int *iters) {
 // level applies, so that the outer level integrates S_F - S_theta.
 static int outer_kick(Traj &T, double eps, bool solve, int *iters) {
   susy_ctx *c = T.c;
+  if (T.pure) return 0;
   RET(sg_record_begin(c));
   for (int n = 0; n < c->p.Nroot; n++) {
     if (solve) RET(traj_cg(T, n, iters));
@@ -847,22 +860,22 @@ static int outer_kick(Traj &T, double eps, bool solve, int *iters) {
   return 0;
 }
 
-static int update_step(Traj &T, int *iters) {
+// One outer step of update_step (update_o.c:80-113): gauge level, solves and
+// fermion kick at the midpoint, gauge level, solves and the closing kick.
+static int outer_step(Traj &T) {
   susy_ctx *c = T.c;
   const double f_eps = c->p.traj_length / (double)c->p.nstep;
   const double g_eps = f_eps / (double)(2.0 * c->p.nstep_gauge);
   double tr;
-  RET(outer_kick(T, f_eps * OM_LAMBDA, false, iters));
-  for (int s = 1; s <= c->p.nstep; s++) {
-    RET(update_gauge_step(T, g_eps, &tr));
-    T.gnorm += tr;
-    if (tr > T.max_gf) T.max_gf = tr;
-    RET(outer_kick(T, f_eps * OM_LAMBDA_MID, true, iters));
-    RET(update_gauge_step(T, g_eps, &tr));
-    T.gnorm += tr;
-    if (tr > T.max_gf) T.max_gf = tr;
-    RET(outer_kick(T, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA), true, iters));
-  }
+  const int s = ++T.step;
+  RET(update_gauge_step(T, g_eps, &tr));
+  T.gnorm += tr;
+  if (tr > T.max_gf) T.max_gf = tr;
+  RET(outer_kick(T, f_eps * OM_LAMBDA_MID, true, &T.iters));
+  RET(update_gauge_step(T, g_eps, &tr));
+  T.gnorm += tr;
+  if (tr > T.max_gf) T.max_gf = tr;
+  RET(outer_kick(T, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA), true, &T.iters));
   return 0;
 }
 
@@ -873,7 +886,7 @@ static int traj_action(Traj &T, double *parts) {
   double tot = a4[0] + (c->p.bmass > IMAG_TOL ? a4[1] : 0.0) +
                (c->p.kappa_u1 > IMAG_TOL ? a4[2] : 0.0);
   double f = 0;
-  for (int n = 0; n < c->p.Nroot; n++) {
+  for (int n = 0; n < (T.pure ? 0 : c->p.Nroot); n++) {
     double fa;
     RET(fermion_action_ptr(c, T.gsrc[n], T.psim[n].data(), &fa));
     f += fa;
@@ -893,56 +906,107 @@ __global__ void k_neg(double2 *__restrict__ y, const double2 *__restrict__ x, si
   if (i < n) y[i] = make_double2(-x[i].x, -x[i].y);
 }
 
-extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
+// Start of a trajectory (update_o.c:127-208): momenta, pseudofermions and
+// their solves, the start action, the saved links and momenta, and the first
+// fermion kick of update_step (update_o.c:74-82).  An unfinished trajectory
+// is abandoned (links and momenta stay as they are).
+extern "C" int susy_traj_begin(susy_ctx *c, int form) {
   if (!c) return SUSY_E_STATE;
-  Traj T;
-  T.c = c;
-  T.form = form & SUSY_FORM_FP32;
-  T.sg = (form & SUSY_FORM_SURROGATE) != 0;
-  if ((form & ~(SUSY_FORM_FP32 | SUSY_FORM_SURROGATE)) != 0) {
-    set_error("susy_update: unknown form bits 0x%x", form);
+  if ((form & ~(SUSY_FORM_FP32 | SUSY_FORM_SURROGATE | SUSY_FORM_PUREGAUGE)) != 0) {
+    set_error("susy_traj_begin: unknown form bits 0x%x", form);
     return SUSY_E_ARG;
   }
-  if (T.sg && !c->sg) {
-    set_error("susy_update: surrogate form without susy_sg_init");
+  const bool sg = (form & SUSY_FORM_SURROGATE) != 0, pure = (form & SUSY_FORM_PUREGAUGE) != 0;
+  if (sg && pure) {
+    set_error("susy_traj_begin: the surrogate form needs the fermions");
+    return SUSY_E_ARG;
+  }
+  if (sg && !c->sg) {
+    set_error("susy_traj_begin: surrogate form without susy_sg_init");
     return SUSY_E_STATE;
   }
+  if (c->p.nstep < 1 || c->p.nstep_gauge < 1) {
+    set_error("susy_traj_begin: nstep and nstep_gauge must be positive");
+    return SUSY_E_ARG;
+  }
+  if (!c->traj) c->traj = new Traj;
+  Traj &T = *c->traj;
+  T = Traj();
+  T.c = c;
+  T.form = form & SUSY_FORM_FP32;
+  T.sg = sg;
+  T.pure = pure;
   const int Nr = c->p.Nroot, No = c->p.Norder;
-  // per-trajectory fields are kept in the context across calls
-  std::vector<double2 *> &pool = c->traj_pool;
-  size_t need = (size_t)Nr * (1 + No);
-  while (pool.size() < need) {
-    double2 *p;
-    RET(dalloc(&p, tf_elems(c)));
-    pool.push_back(p);
+  if (!pure) {
+    // per-trajectory fields are kept in the context across calls
+    std::vector<double2 *> &pool = c->traj_pool;
+    size_t need = (size_t)Nr * (1 + No);
+    while (pool.size() < need) {
+      double2 *p;
+      RET(dalloc(&p, tf_elems(c)));
+      pool.push_back(p);
+    }
+    T.gsrc.resize(Nr);
+    T.psim.resize(Nr);
+    for (int n = 0; n < Nr; n++) {
+      T.gsrc[n] = pool[n * (1 + No)];
+      T.psim[n].assign(pool.begin() + n * (1 + No) + 1, pool.begin() + (n + 1) * (1 + No));
+    }
   }
-  T.gsrc.resize(Nr);
-  T.psim.resize(Nr);
-  for (int n = 0; n < Nr; n++) {
-    T.gsrc[n] = pool[n * (1 + No)];
-    T.psim[n].assign(pool.begin() + n * (1 + No) + 1, pool.begin() + (n + 1) * (1 + No));
-  }
-  int iters = 0;
   RET(ranmom(c));
-  for (int n = 0; n < Nr; n++) {
+  for (int n = 0; n < (pure ? 0 : Nr); n++) {
     double t0 = g_verbose ? now_s() : 0;
-    int before = iters;
-    RET(grsource(c, T.gsrc[n], T.psim[n].data(), T.form, &iters));
+    int before = T.iters;
+    RET(grsource(c, T.gsrc[n], T.psim[n].data(), T.form, &T.iters));
     if (g_verbose) {
-      fprintf(stderr, "susy: heatbath root %d: %d iterations, %.3f s\n", n, iters - before,
+      fprintf(stderr, "susy: heatbath root %d: %d iterations, %.3f s\n", n, T.iters - before,
               now_s() - t0);
       fflush(stderr);
     }
   }
-  for (int n = 0; n < Nr; n++) RET(traj_cg(T, n, &iters));
-  double p0[6], p1[6];
-  RET(traj_action(T, p0));
+  for (int n = 0; n < (pure ? 0 : Nr); n++) RET(traj_cg(T, n, &T.iters));
+  RET(traj_action(T, T.p0));
   const size_t mat5 = (size_t)NUMLINK * NN * c->V;
   HIPCHK(hipMemcpyAsync(c->d_old_links, c->d_links, sizeof(double2) * mat5,
                         hipMemcpyDeviceToDevice, c->s));
   HIPCHK(hipMemcpyAsync(c->d_old_mom, c->d_mom, sizeof(double2) * mat5,
                         hipMemcpyDeviceToDevice, c->s));
-  RET(update_step(T, &iters));
+  RET(outer_kick(T, c->p.traj_length / (double)c->p.nstep * OM_LAMBDA, false, &T.iters));
+  T.open = true;
+  return 0;
+}
+
+// One outer MD step of the open trajectory; returns the steps done so far.
+extern "C" int susy_traj_step(susy_ctx *c) {
+  if (!c) return SUSY_E_STATE;
+  if (!c->traj || !c->traj->open || c->traj->step >= c->p.nstep) {
+    set_error("susy_traj_step: no open trajectory with steps left");
+    return SUSY_E_STATE;
+  }
+  Traj &T = *c->traj;
+  int r = outer_step(T);
+  if (r < 0) {
+    T.open = false;
+    return r;
+  }
+  return T.step;
+}
+
+// End of the trajectory (update_o.c:209-282): the end action (the last
+// solves belong to the final links), the Metropolis test with one draw of the
+// node generator, and on rejection the saved links, reversed momenta and
+// recomputed link fields.  Returns 1 (accepted) or 0 (rejected).
+extern "C" int susy_traj_end(susy_ctx *c, double *stats) {
+  if (!c) return SUSY_E_STATE;
+  if (!c->traj || !c->traj->open || c->traj->step != c->p.nstep) {
+    set_error("susy_traj_end: the open trajectory has not made its %d steps", c->p.nstep);
+    return SUSY_E_STATE;
+  }
+  Traj &T = *c->traj;
+  T.open = false;
+  const int Nr = c->p.Nroot;
+  double p1[6];
+  const double *p0 = T.p0;
   RET(traj_action(T, p1));
   double change = p1[5] - p0[5];
   if (fabs(change) > 1e20 || change != change) change = 1.0e20;
@@ -951,6 +1015,7 @@ extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
   if (exp(-change) < xr) {
     accept = 0;
     if (c->p.traj_length > 0.0) {
+      const size_t mat5 = (size_t)NUMLINK * NN * c->V;
       HIPCHK(hipMemcpyAsync(c->d_links, c->d_old_links, sizeof(double2) * mat5,
                             hipMemcpyDeviceToDevice, c->s));
       hipLaunchKernelGGL(k_neg, dim3((mat5 + 255) / 256), dim3(256), 0, c->s, c->d_mom,
@@ -962,15 +1027,21 @@ extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
   HIPCHK(hipStreamSynchronize(c->s));
   if (stats) {
     stats[0] = p0[5]; stats[1] = p1[5]; stats[2] = change; stats[3] = accept;
-    stats[4] = iters;
+    stats[4] = T.iters;
     stats[5] = T.gnorm / (2.0 * c->p.nstep); stats[6] = T.max_gf;
     for (int n = 0; n < 2; n++) {
-      stats[7 + 2 * n] = n < Nr ? T.fnorm[n] / (2.0 * c->p.nstep) : 0;
-      stats[8 + 2 * n] = n < Nr ? T.max_ff[n] : 0;
+      stats[7 + 2 * n] = n < Nr && !T.pure ? T.fnorm[n] / (2.0 * c->p.nstep) : 0;
+      stats[8 + 2 * n] = n < Nr && !T.pure ? T.max_ff[n] : 0;
     }
     for (int k = 0; k < 6; k++) { stats[11 + k] = p0[k]; stats[17 + k] = p1[k]; }
   }
   return accept;
+}
+
+extern "C" int susy_update(susy_ctx *c, int form, double *stats) {
+  RET(susy_traj_begin(c, form));
+  for (int s = 1; s <= c->p.nstep; s++) RET(susy_traj_step(c));
+  return susy_traj_end(c, stats);
 }
 
 // =====================================================================

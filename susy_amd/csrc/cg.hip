@@ -160,10 +160,10 @@ __global__ void k_cgs_beta(CgState *st, const double *__restrict__ cd) {
                s0 = st->shifts[0];
   vm_drain();
   const double bi0 = -rsq / cdv;
-  st->iteration = it + 1;
+  st_keep(&st->iteration, it + 1);
   st->bi[0] = bi0;
   hold(bi0);
-  st->zip1[0] = 1;
+  st_keep(&st->zip1[0], 1.0);
   for (int j = 1; j < nsh; j++) {
     const int cv = st->conv[j];
     const double zij = st->zi[j], zim1j = st->zim1[j], sj = st->shifts[j];

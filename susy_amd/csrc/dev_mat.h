@@ -24,11 +24,16 @@ __device__ __forceinline__ void mload(Mat m, const double2 *p, int V) {
 #pragma unroll
     for (int j = 0; j < NCOL; j++) m[i][j] = p[(i * NCOL + j) * V];
 }
+// (each stored value is held until the store has read its registers)
 __device__ __forceinline__ void mstore(double2 *p, const Mat m, int V) {
 #pragma unroll
   for (int i = 0; i < NCOL; i++)
 #pragma unroll
     for (int j = 0; j < NCOL; j++) p[(i * NCOL + j) * V] = m[i][j];
+#pragma unroll
+  for (int i = 0; i < NCOL; i++)
+#pragma unroll
+    for (int j = 0; j < NCOL; j++) hold(m[i][j]);
 }
 __device__ __forceinline__ void mzero(Mat m) {
 #pragma unroll
@@ -73,7 +78,7 @@ __device__ __forceinline__ void block_partial(double v, double *partial, int slo
   if (threadIdx.x == 0) {
     double t = 0;
     for (int w = 0; w < (int)(blockDim.x / 64); w++) t += red[w][slot];
-    partial[slot * slot_stride + blockIdx.x] = t;
+    st_keep(partial + slot * slot_stride + blockIdx.x, t);
   }
   __syncthreads();
 }

@@ -61,7 +61,7 @@ k_det_inv(const double2 *__restrict__ U, double2 *__restrict__ det,
     d = cmul(d, m[i][i]);
     if (piv[i] != i) d = make_double2(-d.x, -d.y);
   }
-  det[a * V + s] = d;
+  st_keep(det + a * V + s, d);
   // inverse: solve L U X = P^T I column by column
   Mat x;
 #pragma unroll
@@ -111,16 +111,16 @@ k_plaqdet(const double2 *__restrict__ det, const int *__restrict__ nbr,
     int xa = nbr[g_fwd(a) * V + s];
     for (int b = 0; b < NUMLINK; b++) {
       if (a == b) {
-        pd[(a * NUMLINK + b) * V + s] = make_double2(0, 0);
-        zw[(a * NUMLINK + b) * V + s] = make_double2(0, 0);
+        st_keep(pd + (a * NUMLINK + b) * V + s, make_double2(0, 0));
+        st_keep(zw + (a * NUMLINK + b) * V + s, make_double2(0, 0));
         continue;
       }
       int xb = nbr[g_fwd(b) * V + s];
       double2 t = cmul(cconjd(det[a * V + s]), det[b * V + s]);
       t = cmul(t, cconjd(det[b * V + xa]));
       t = cmul(det[a * V + xb], t);
-      pd[(a * NUMLINK + b) * V + s] = t;
-      zw[(a * NUMLINK + b) * V + s] = cmul(t, cconjd(make_double2(t.x - 1.0, t.y)));
+      st_keep(pd + (a * NUMLINK + b) * V + s, t);
+      st_keep(zw + (a * NUMLINK + b) * V + s, cmul(t, cconjd(make_double2(t.x - 1.0, t.y))));
     }
   }
 }
@@ -184,7 +184,7 @@ k_fmunu(const double2 *__restrict__ U, const int *__restrict__ nbr,
 template <typename T>
 __global__ void k_demote(const double2 *__restrict__ in, T *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+  if (i < n) st_keep(out + i, make_float2((float)in[i].x, (float)in[i].y));
 }
 
 int fields_recompute(susy_ctx *c) {
@@ -224,7 +224,7 @@ __global__ void k_update_u(double2 *__restrict__ U, const double2 *__restrict__ 
     double2 u = U[i], m = mom[i];
     u.x += eps * m.x;
     u.y += eps * m.y;
-    U[i] = u;
+    st_keep(U + i, u);
   }
 }
 
@@ -324,7 +324,7 @@ k_gauge_force(const double2 *__restrict__ U, const double2 *__restrict__ dmu,
         double2 m = mp[(i * NCOL + k) * V];
         m.x -= kappa_eps * f[k][i].x;
         m.y += kappa_eps * f[k][i].y;
-        mp[(i * NCOL + k) * V] = m;
+        st_keep(mp + (i * NCOL + k) * V, m);
       }
     nrm = retrace_dag(f, f);
   }
@@ -365,7 +365,7 @@ k_det_force(const double2 *__restrict__ uinv, const double2 *__restrict__ zw,
         double2 m = mp[(i * NCOL + k) * V];
         m.x -= eps * f[k][i].x;
         m.y += eps * f[k][i].y;
-        mp[(i * NCOL + k) * V] = m;
+        st_keep(mp + (i * NCOL + k) * V, m);
       }
     nrm = retrace_dag(f, f);
   }

@@ -71,8 +71,8 @@ k_force_prep(const double2 *__restrict__ sol, const double2 *__restrict__ psol,
     ts.x += v.x; ts.y += v.y;
     tp.x += w.x; tp.y -= w.y;
   }
-  aux[s] = ts;
-  aux[V + s] = tp;
+  st_keep(aux + s, ts);
+  st_keep(aux + V + s, tp);
   for (int a = 0; a < NUMLINK; a++) {
     const double2 *u = uinv + a * M + s;
     const double2 *ps = sol + (1 + a) * M + s, *pp = psol + (1 + a) * M + s;
@@ -87,8 +87,8 @@ k_force_prep(const double2 *__restrict__ sol, const double2 *__restrict__ psol,
         t1.x += x.x * y.x - x.y * y.y; t1.y += x.x * y.y + x.y * y.x;
         t2.x += x.x * z.x - x.y * z.y; t2.y += x.x * z.y + x.y * z.x;
       }
-    aux[(2 + a) * V + s] = t1;
-    aux[(7 + a) * V + s] = t2;
+    st_keep(aux + (2 + a) * V + s, t1);
+    st_keep(aux + (7 + a) * V + s, t2);
   }
 }
 
@@ -255,7 +255,7 @@ k_force_assemble(const double2 *__restrict__ S, const double2 *__restrict__ P,
         v.x += o.x;
         v.y += o.y;
       }
-      fo[(i * NCOL + j) * V] = v;
+      st_keep(fo + (i * NCOL + j) * V, v);
     }
 }
 
@@ -274,7 +274,7 @@ k_force_apply(const double2 *__restrict__ F, double2 *__restrict__ mom, double e
       double2 f = F[k], m = mom[k];
       m.x -= eps * f.x;
       m.y -= eps * f.y;
-      mom[k] = m;
+      st_keep(mom + k, m);
       nrm += f.x * f.x + f.y * f.y;
     }
   }

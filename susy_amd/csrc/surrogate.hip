@@ -261,8 +261,8 @@ k_sg_features(const double2 *__restrict__ U, const int *__restrict__ base,
       mmul_into(acc, f);
     }
     double2 tr = mtrace(acc);
-    phi[(size_t)(2 * k) * V + y] = (float)(tr.x / NCOL);
-    phi[(size_t)(2 * k + 1) * V + y] = (float)(tr.y / NCOL);
+    ST(phi[(size_t)(2 * k) * V + y], (float)(tr.x / NCOL));
+    ST(phi[(size_t)(2 * k + 1) * V + y], (float)(tr.y / NCOL));
   }
 #pragma unroll 1
   for (int a = 0; a < NUMLINK; a++) {
@@ -272,7 +272,7 @@ k_sg_features(const double2 *__restrict__ U, const int *__restrict__ base,
       double2 v = p[(size_t)e * V];
       s += v.x * v.x + v.y * v.y;
     }
-    phi[(size_t)(60 + a) * V + y] = (float)(s / NCOL - 1.0);
+    ST(phi[(size_t)(60 + a) * V + y], (float)(s / NCOL - 1.0));
   }
 }
 
@@ -301,8 +301,8 @@ k_sg_jvp(const double2 *__restrict__ U, const double2 *__restrict__ R,
       tot.x += tr.x;
       tot.y += tr.y;
     }
-    v[(size_t)(2 * k) * V + y] = (float)(tot.x / NCOL);
-    v[(size_t)(2 * k + 1) * V + y] = (float)(tot.y / NCOL);
+    ST(v[(size_t)(2 * k) * V + y], (float)(tot.x / NCOL));
+    ST(v[(size_t)(2 * k + 1) * V + y], (float)(tot.y / NCOL));
   }
 #pragma unroll 1
   for (int a = 0; a < NUMLINK; a++) {
@@ -312,7 +312,7 @@ k_sg_jvp(const double2 *__restrict__ U, const double2 *__restrict__ R,
       double2 u = p[(size_t)e * V], w = q[(size_t)e * V];
       s += u.x * w.x + u.y * w.y;
     }
-    v[(size_t)(60 + a) * V + y] = (float)(2.0 * s / NCOL);
+    ST(v[(size_t)(60 + a) * V + y], (float)(2.0 * s / NCOL));
   }
 }
 
@@ -353,7 +353,7 @@ k_sg_conv1(const unsigned short *__restrict__ wbf, const float *__restrict__ the
   float z[SG_NC];
   conv_site<SG_NF, SG_NC, false>(wbf + SG_W1, phi, nbr, s, V, z);
 #pragma unroll
-  for (int o = 0; o < SG_NC; o++) h1[(size_t)o * V + s] = tanhf(z[o] + theta[SG_B1 + o]);
+  for (int o = 0; o < SG_NC; o++) ST(h1[(size_t)o * V + s], tanhf(z[o] + theta[SG_B1 + o]));
 }
 
 // h2 = tanh(b2 + conv(W2, h1)); gz2 = (1 - h2^2) w3; partial of sum w3 . h2
@@ -369,8 +369,8 @@ k_sg_conv2(const unsigned short *__restrict__ wbf, const float *__restrict__ the
 #pragma unroll
     for (int o = 0; o < SG_NC; o++) {
       const float h = tanhf(z[o] + theta[SG_B2 + o]), w = bf2f(wbf[SG_W3 + o]);
-      h2[(size_t)o * V + s] = h;
-      gz2[(size_t)o * V + s] = (1.f - h * h) * w;
+      ST(h2[(size_t)o * V + s], h);
+      ST(gz2[(size_t)o * V + s], (1.f - h * h) * w);
       act += (double)w * (double)h;
     }
   }
@@ -389,7 +389,7 @@ k_sg_back1(const unsigned short *__restrict__ wbf, const float *__restrict__ gz2
 #pragma unroll
   for (int i = 0; i < SG_NC; i++) {
     const float h = h1[(size_t)i * V + s];
-    gz1[(size_t)i * V + s] = (1.f - h * h) * g[i];
+    ST(gz1[(size_t)i * V + s], (1.f - h * h) * g[i]);
   }
 }
 
@@ -402,7 +402,7 @@ k_sg_back0(const unsigned short *__restrict__ wbf, const float *__restrict__ gz1
   float g[SG_NF];
   conv_site<SG_NF, SG_NC, true>(wbf + SG_W1, gz1, nbr, s, V, g);
 #pragma unroll
-  for (int i = 0; i < SG_NF; i++) gphi[(size_t)i * V + s] = g[i];
+  for (int i = 0; i < SG_NF; i++) ST(gphi[(size_t)i * V + s], g[i]);
 }
 
 // F = dS/dUbar on link (x, a)
@@ -456,8 +456,8 @@ k_sg_force(const double2 *__restrict__ U, const int *__restrict__ diff,
 #pragma unroll
   for (int e = 0; e < NN; e++) {
     const double2 w = u[(size_t)e * V];
-    out[(size_t)e * V] = make_double2(acc[e / NCOL][e % NCOL].x + gs * w.x,
-                                      acc[e / NCOL][e % NCOL].y + gs * w.y);
+    st_keep(out + (size_t)e * V, make_double2(acc[e / NCOL][e % NCOL].x + gs * w.x,
+                                              acc[e / NCOL][e % NCOL].y + gs * w.y));
   }
 }
 
@@ -469,7 +469,7 @@ __global__ void k_sg_apply(const double2 *__restrict__ F, double2 *__restrict__ 
     double2 f = F[i], m = mom[i];
     m.x -= eps * f.x;
     m.y -= eps * f.y;
-    mom[i] = m;
+    ST(mom[i], m);
   }
 }
 
@@ -481,7 +481,7 @@ k_sg_resid(const double2 *__restrict__ F, const double2 *__restrict__ Ft,
   for (size_t i = (size_t)blockIdx.x * FB + threadIdx.x; i < n; i += (size_t)gridDim.x * FB) {
     const double2 f = F[i], t = Ft[i];
     const double2 r = make_double2(f.x - t.x, f.y - t.y);
-    R[i] = r;
+    ST(R[i], r);
     r2 += r.x * r.x + r.y * r.y;
     f2 += t.x * t.x + t.y * t.y;
   }
@@ -501,8 +501,8 @@ k_sg_tan1(const unsigned short *__restrict__ wbf, const float *__restrict__ v,
 #pragma unroll
   for (int o = 0; o < SG_NC; o++) {
     const float h = h1[(size_t)o * V + s];
-    z1d[(size_t)o * V + s] = z[o];
-    h1d[(size_t)o * V + s] = (1.f - h * h) * z[o];
+    ST(z1d[(size_t)o * V + s], z[o]);
+    ST(h1d[(size_t)o * V + s], (1.f - h * h) * z[o]);
   }
 }
 
@@ -519,9 +519,9 @@ k_sg_tan2(const unsigned short *__restrict__ wbf, const float *__restrict__ h1d,
 #pragma unroll
   for (int o = 0; o < SG_NC; o++) {
     const float h = h2[(size_t)o * V + s], a2 = 1.f - h * h, w = bf2f(wbf[SG_W3 + o]);
-    h2d[(size_t)o * V + s] = a2 * z[o];
-    az2d[(size_t)o * V + s] = a2 * w;
-    az2[(size_t)o * V + s] = -2.f * h * a2 * w * z[o];
+    ST(h2d[(size_t)o * V + s], a2 * z[o]);
+    ST(az2d[(size_t)o * V + s], a2 * w);
+    ST(az2[(size_t)o * V + s], -2.f * h * a2 * w * z[o]);
   }
 }
 
@@ -541,8 +541,8 @@ k_sg_tan3(const unsigned short *__restrict__ wbf, const float *__restrict__ az2d
   for (int i = 0; i < SG_NC; i++) {
     const float h = h1[(size_t)i * V + s], a1 = 1.f - h * h;
     const float ah1 = gp[i] - 2.f * h * gd[i] * z1d[(size_t)i * V + s];
-    az1d[(size_t)i * V + s] = a1 * gd[i];
-    az1[(size_t)i * V + s] = a1 * ah1;
+    ST(az1d[(size_t)i * V + s], a1 * gd[i]);
+    ST(az1[(size_t)i * V + s], a1 * ah1);
   }
 }
 
@@ -619,14 +619,14 @@ k_sg_wpart(SgGradIn in, const int *__restrict__ nbr, double *__restrict__ part,
 #pragma unroll
   for (int k = 0; k < WG_SLOTS; k++) {
     const int p = tid + k * SGB;
-    if (p < WG_PT) part[((size_t)chunk * SG_NT + t) * WG_PT + p] = acc[k];
+    if (p < WG_PT) ST(part[((size_t)chunk * SG_NT + t) * WG_PT + p], acc[k]);
   }
   if (t == 0 && tid < WG_NB) {
     const float *f = tid < SG_NC ? in.az1 : tid < 2 * SG_NC ? in.az2 : in.h2d;
     const int o = tid % SG_NC;
     double s = 0;
     for (int x = x0; x < x1; x++) s += f[(size_t)o * V + x];
-    bpart[(size_t)chunk * WG_NB + tid] = s;
+    ST(bpart[(size_t)chunk * WG_NB + tid], s);
   }
 }
 
@@ -657,24 +657,26 @@ __global__ void k_sg_adam(float *__restrict__ theta, float *__restrict__ m,
   if (p >= SG_NP) return;
   const float g = (float)(gacc[p] * scale);
   const float mm = 0.9f * m[p] + 0.1f * g, vv = 0.999f * v[p] + 0.001f * g * g;
-  m[p] = mm;
-  v[p] = vv;
+  ST(m[p], mm);
+  ST(v[p], vv);
   const float th = theta[p] - lr * (mm / b1c) / (sqrtf(vv / b2c) + 1e-8f);
-  theta[p] = th;
-  wbf[p] = f2bf(th);
-  gacc[p] = 0.0;
+  ST(theta[p], th);
+  ST(wbf[p], f2bf(th));
+  ST(gacc[p], 0.0);
 }
 
 __global__ void k_sg_round(const float *__restrict__ theta, unsigned short *__restrict__ wbf) {
   const int p = blockIdx.x * SGB + threadIdx.x;
-  if (p < SG_NP) wbf[p] = f2bf(theta[p]);
+  if (p < SG_NP) ST(wbf[p], f2bf(theta[p]));
 }
 
 __global__ void k_sg_addf(double2 *__restrict__ y, const double2 *__restrict__ x, size_t n) {
   size_t i = (size_t)blockIdx.x * SGB + threadIdx.x;
   if (i < n) {
-    y[i].x += x[i].x;
-    y[i].y += x[i].y;
+    double2 v = y[i], w = x[i];
+    v.x += w.x;
+    v.y += w.y;
+    st_keep(y + i, v);
   }
 }
 

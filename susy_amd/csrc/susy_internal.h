@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/susy_hip.h"
@@ -103,14 +104,20 @@ __device__ __forceinline__ void hold(double2 v) {
 __device__ __forceinline__ void hold(float2 v) {
   asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
 }
-__device__ __forceinline__ void st_keep(double2 *p, double2 v) {
-  *p = v;
-  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+__device__ __forceinline__ void hold(int v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
+__device__ __forceinline__ void hold(unsigned v) { asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory"); }
+__device__ __forceinline__ void hold(unsigned short v) {
+  asm volatile("s_waitcnt expcnt(0)" ::"v"(v) : "memory");
 }
-__device__ __forceinline__ void st_keep(float2 *p, float2 v) {
+// Store one value and keep its registers until the store has read them.
+// Every global store of the library goes through this (or through a batch of
+// stores followed by hold() of each stored value).
+template <typename T> __device__ __forceinline__ void st_keep(T *p, T v) {
   *p = v;
-  asm volatile("s_waitcnt expcnt(0)" ::"v"(v.x), "v"(v.y) : "memory");
+  hold(v);
 }
+// lhs = rhs through st_keep, rhs converted to the element type of lhs
+#define ST(lhs, rhs) st_keep(&(lhs), (std::remove_reference_t<decltype(lhs)>)(rhs))
 
 template <typename R> struct CT;
 template <> struct CT<double> { typedef double2 t; };
@@ -193,6 +200,7 @@ double prn_next(Prn *p);
 // ---------------------------------------------------------------------
 // Context
 struct SgState;   // learned surrogate (surrogate.hip)
+struct Traj;      // trajectory in progress (capi.hip)
 
 struct susy_ctx {
   susy_params p;
@@ -253,6 +261,7 @@ struct susy_ctx {
   int site_order32 = 2;        // the same for the FP32 operator
   int pf_handle = -1;          // two Twist_Fermion handles of the Pfaffian (susy_pfaffian_dev)
   SgState *sg = nullptr;       // learned surrogate state (susy_sg_init)
+  Traj *traj = nullptr;        // trajectory between susy_traj_begin and susy_traj_end
   int prof_every = 0;
   long prof_launch[2] = {0, 0};
   std::vector<hipEvent_t> prof_ev[2];
@@ -314,6 +323,8 @@ int cg_multi(susy_ctx *c, const double2 *src, double2 *const *psim, int nshift,
              int form);
 
 size_t tf_elems(const susy_ctx *c);   // NTF * NN * V complex
+
+void traj_free(susy_ctx *c);           // (capi.hip)
 
 // learned surrogate (surrogate.hip)
 void sg_free(susy_ctx *c);

@@ -6,12 +6,15 @@ A store reads its data registers some cycles after it issues; an
 overwrite scheduled right behind it can reach memory instead of the stored
 value.  This produced sporadic wrong 64-byte groups in the operator's FP64
 output (repeated DSq and multi-shift CG runs at U(3) 8^3x16 were not
-bitwise repeatable; a wait after each store made them so).  The operator
-and CG sources store through helpers that hold the data until the wait
-(bstore, st_keep, hold); this module compiles them to device assembly with
-the build's flags and lists the kernels that still overwrite store data.
+bitwise repeatable; a wait after each store made them so).  Every
+source of the library stores through helpers that hold the data until the
+wait (bstore + hold, st_keep, ST); this module compiles each source to device
+assembly with the build's flags and lists the kernels that still overwrite
+store data.  The scan is linear through each kernel: a store's registers stay
+"live" across labels and branches until a full wait, so an overwrite on any
+later path is reported.
 
-    python -m susy_amd.store_check        (prints the flagged kernels)
+    python tests/store_check.py [NCOL]    (prints the flagged kernels)
 """
 import collections
 import concurrent.futures as cf
@@ -21,16 +24,17 @@ import subprocess
 import sys
 import tempfile
 
-from . import build as B
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from susy_amd import build as B  # noqa: E402
 
-CHECKED = ("dslash.hip", "cg.hip")
+CHECKED = tuple(B.SOURCES)
 FUNC = re.compile(r"^(_Z\S+):\s")
 STORE = re.compile(r"^\s*(buffer_store_\w+|global_store_\w+|flat_store_\w+)\s+(v\[\d+:\d+\]|v\d+),"
                    r"\s*(v\[\d+:\d+\]|v\d+)")
 WRITE = re.compile(r"^\s*(v_\w+|buffer_load\w*|global_load\w*|flat_load\w*|ds_read\w*|"
                    r"ds_bpermute\w*|ds_swizzle\w*)\s+(v\[\d+:\d+\]|v\d+)")
 FULL_WAIT = re.compile(r"^\s*s_waitcnt\b.*(expcnt\(0\)|vmcnt\(0\))")
-BLOCK_END = re.compile(r"^\.LBB|^\s*s_(cbranch|branch|endpgm)")
+BLOCK_END = re.compile(r"^\s*s_endpgm")
 
 
 def _regs(tok):
