@@ -3,36 +3,54 @@
 Workload: U(4) on a 16^3 x 32 lattice, lambda = 2.0, bmass = 0.4, G = 0.05,
 kappa_u1 = 0, fmass = 0, Nroot = 2, Norder = 15, antiperiodic temporal
 fermion BC, CG stopping at |r|^2 <= 1e-10 |src|^2 (error_per_site 1e-5).
-Synthetic data: cold start (unit links); momenta and pseudofermions from the
-reference's generators seeded with 1234.
+Trajectories of length 1 with nstep = 20 outer (fermion) steps and
+nstep_gauge = 10 gauge steps inside each half step (MD step 0.05).
 
-One step = one full RHMC trajectory: heatbath (two multi-shift solves), the
-two-level Omelyan integrator with fermion forces (two multi-shift solves per
-force) and nstep_gauge gauge steps inside each, and the Metropolis test on
-the action recomputed with two more solves.  The default trajectory length
-is 0.25 with nstep = 5 (MD step 0.05, the step of a unit-length trajectory
-with nstep = 20): one unit-length trajectory on this lattice takes several
-minutes of GPU time (a thermalising configuration needs 2000-7000 CG
-iterations per solve), so `--traj-length 1 --nstep 20` is available but not
-the default.
+Start: a thermalised configuration.  A 16^3 x 32 lattice needs tens of unit
+trajectories from a cold start before its actions and solver costs settle,
+far more than a benchmark run can spend, so the start is an 8^3 x 16
+configuration with the same couplings thermalised by this library's own RHMC
+(committed as susy_amd/configs/therm_u4_8x8x8x16.milc; made by
+`python -m susy_amd.start_config`) tiled 2 x 2 x 2 x 2 onto 16^3 x 32.
+Momenta and pseudofermions come from the reference's generators seeded with
+1234.  Every form starts from the same links with the same seed, so their
+first trajectories have identical momenta and pseudofermions.
 
-Forms: the headline FP32 form (FP32 operator and multi-shift CG with FP64
-reliable updates, FP64 forces, actions and dH), the plain FP64 form and the
-FP32 form with the learned surrogate force on the inner MD level, all from
-the same links with the same generator seed (identical momenta and
-pseudofermions), so their first trajectories differ only by arithmetic and
-by the surrogate.  The surrogate is trained in the run itself: the plain
-FP64 trajectory records (links, fermion force) at every outer kick, the
-network is trained on them with the library's own forward and backward
-kernels (Adam, `--sg-steps`), and its relative force error on the last two
-recorded pairs (held out) is reported with the form.  The surrogate form is
-timed but is not the headline `value`: it is reported beside it.
+One step = one outer MD step of the trajectory (update_o.c:80-113): the gauge
+level (2 x nstep_gauge Omelyan gauge steps), 2 x Nroot multi-shift solves and
+2 x Nroot fermion-force kicks.  A trajectory is nstep = 20 such steps plus its
+start (heatbath: 2 x Nroot solves; first kick) and end (action, Metropolis),
+so `--steps 20` times exactly one trajectory's worth of work: the timed
+steps run on from the warm-up steps of the first trajectory into the next,
+and the end of one trajectory and the start of the next fall inside the
+timed region.
+
+Forms:
+  * FP32 + surrogate (headline `value`): FP32 operator and multi-shift CG with
+    FP64 reliable updates, FP64 forces, actions and dH, and the learned
+    surrogate force on the inner (gauge) MD level; `--warmup` untimed steps,
+    then `--steps` timed steps, trajectories completed and Metropolis-tested
+    as they end.
+  * FP32 (no surrogate) and plain FP64: the start of one trajectory and
+    `--fp32-steps` / `--fp64-steps` timed outer steps from the same links and
+    seed; their trajectory time is start + nstep x the mean step time.
+The surrogate is trained in the run itself, on the (links, fermion force)
+pairs recorded at every outer kick of the FP32 and FP64 legs (2 held out),
+with the library's own forward and backward kernels (Adam, `--sg-steps`).
+
+value = accepted trajectories per hour of the FP32+surrogate form =
+3600 x acceptance x (timed steps / nstep) / timed seconds, with the
+acceptance of the trajectories that ended in the run (reported with their
+count and dH).  The ratio to the plain FP64 form's trajectories per hour is
+reported beside it; acceptance rates from longer runs of all three forms are
+not part of this command.
 
 `--kernels` runs only a short operator workload (5 field copies for the
 memory-counter calibration, then 10 launches of the operator in each
 precision) for counter runs under rocprofv3.
 
-Prints one JSON line.  Only one GPU is supported.
+Prints one JSON line (progress lines go to stderr).  Only one GPU is
+supported.
 """
 import argparse
 import json
@@ -50,10 +68,11 @@ PEAK_HBM_GBS = 8000.0
 WORKLOAD = "u4_16x16x16x32"
 # counter summary of the operator kernel at this workload (HBM bytes per launch)
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"k_dslash_fp32_{WORKLOAD}_counters.txt")
+SEED = 1234
 
 
 def workload(traj_length, nstep, nstep_gauge):
-    return dict(dims=DIMS, PBC=-1, iseed=1234, Nroot=2, Norder=15, lam=2.0, kappa_u1=0.0,
+    return dict(dims=DIMS, PBC=-1, iseed=SEED, Nroot=2, Norder=15, lam=2.0, kappa_u1=0.0,
                 bmass=0.4, fmass=0.0, G=0.05, niter=5000, error_per_site=1e-5,
                 traj_length=traj_length, nstep=nstep, nstep_gauge=nstep_gauge)
 
@@ -75,35 +94,271 @@ def measured_traffic():
         return None
 
 
+def say(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def finite_or_none(x):
     x = float(x)
     return x if x == x and abs(x) < 1e300 else None
 
 
-def run_form(lat, form, steps, start_links, prof_every):
-    """`steps` trajectories from start_links with freshly seeded generators"""
-    lat.set_links(start_links)
-    lat.reseed(1234)
+def sync_time():
+    return time.perf_counter()
+
+
+def partial_form(lat, susy_amd, form, name, start, nsteps, prof_every):
+    """start of one trajectory and nsteps timed outer steps"""
+    lat.set_links(start)
+    lat.reseed(SEED)
     lat.prof_enable(prof_every)
     t0 = time.perf_counter()
-    stats = []
-    for k in range(steps):
-        stats.append(lat.update(form))
-        print(f"bench: form {form} trajectory {k}: dH {stats[-1][2]:.4g} accept "
-              f"{int(stats[-1][3])} iters {int(stats[-1][4])} "
-              f"t {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    t1 = time.perf_counter()
-    prec = form & 1
-    avg_ms, launches, sampled = lat.prof_read(prec)
+    lat.traj_begin(form)
+    t_begin = time.perf_counter() - t0
+    say(f"{name}: start of trajectory {t_begin:.2f} s")
+    steps = []
+    for k in range(nsteps):
+        t = time.perf_counter()
+        lat.traj_step()
+        steps.append(time.perf_counter() - t)
+        say(f"{name}: step {k + 1} {steps[-1]:.2f} s")
+    prof = lat.prof_read(form & 1)
     lat.prof_enable(0)
-    return stats, t1 - t0, (avg_ms, launches, sampled), lat.plaquette()
+    return {"s_begin": t_begin, "s_steps": steps, "dslash_ms": prof[0]}
 
 
-def cpu_baseline(fp64_stats, nforce_solves):
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed outer MD steps of the FP32+surrogate form")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed outer steps before them")
+    ap.add_argument("--fp32-steps", type=int, default=2, help="timed steps, FP32 form")
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed steps, plain FP64 form")
+    ap.add_argument("--traj-length", type=float, default=1.0)
+    ap.add_argument("--nstep", type=int, default=20)
+    ap.add_argument("--nstep-gauge", type=int, default=10)
+    ap.add_argument("--prof-every", type=int, default=8,
+                    help="time every n-th operator kernel launch with HIP events")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels", action="store_true", help="operator launches only")
+    ap.add_argument("--sg-steps", type=int, default=200, help="Adam steps of the surrogate")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not an implementation",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("bench.py: --steps must be positive and --warmup non-negative")
+
+    import numpy as np
+
+    import susy_amd
+    t_job = time.perf_counter()
+    lat = susy_amd.Lattice(NCOL, **workload(args.traj_length, args.nstep, args.nstep_gauge))
+    if args.kernels:
+        kernels_only(lat)
+        return
+    susy_amd.load(NCOL).susy_set_verbose(1)   # one progress line per solve (stderr)
+    V, nstep = lat.V, args.nstep
+
+    # ---- start configuration: thermalised 8^3 x 16 links tiled onto 16^3 x 32
+    from susy_amd import start_config
+    lat.set_links(start_config.tiled_start(susy_amd, DIMS))
+    start = lat.get_links()
+    plaq_start = lat.plaquette()
+    say(f"start: tiled thermalised configuration, plaquette {plaq_start[0]:.6f}")
+
+    # ---- isolated operator timing (HIP events, 20 launches each)
+    h = lat.tf_alloc(2)
+    rng = np.random.default_rng(5)
+    lat.tf_upload(h, (rng.standard_normal(lat.tf_shape())
+                      + 1j * rng.standard_normal(lat.tf_shape())) / np.sqrt(2))
+    iso = {nm: lat.time_fermion_op(h, h + 1, form=f, reps=20)
+           for nm, f in (("fp64", 0), ("fp32", 1))}
+    say(f"isolated D: FP64 {iso['fp64']:.3f} ms, FP32 {iso['fp32']:.3f} ms")
+
+    # ---- FP32 and FP64 forms: start + a few steps each, recording force pairs
+    npair_cap = 2 * (args.fp32_steps + args.fp64_steps) + 2
+    lat.sg_init(seed=1, scale=1.0)
+    lat.sg_record(npair_cap)
+    f32 = partial_form(lat, susy_amd, susy_amd.FORM_FP32, "fp32", start, args.fp32_steps,
+                       args.prof_every)
+    f64 = partial_form(lat, susy_amd, susy_amd.FORM_FP64, "fp64", start, args.fp64_steps,
+                       args.prof_every)
+    lat.sg_record(0)
+
+    # ---- surrogate training on the recorded pairs
+    npairs = lat.sg_pairs()
+    ntest = 2 if npairs > 4 else 0
+    held = list(range(npairs - ntest, npairs))
+    before = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
+    t = time.perf_counter()
+    hist = lat.sg_train(0, npairs - ntest, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
+    t_train = time.perf_counter() - t
+    after = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
+    say(f"surrogate: {npairs} pairs, {args.sg_steps} Adam steps {t_train:.1f} s, held-out "
+        f"force error^2 {before} -> {after}")
+
+    # ---- headline: FP32 + surrogate, warmup steps then timed steps
+    form = susy_amd.FORM_FP32 | susy_amd.FORM_SURROGATE
+    lat.set_links(start)
+    lat.reseed(SEED)
+    trajs = []
+    done = [0]   # outer steps of the open trajectory
+
+    def one_step():
+        if done[0] == 0:
+            lat.traj_begin(form)
+        lat.traj_step()
+        done[0] += 1
+        if done[0] == nstep:
+            st = lat.traj_end()
+            plaq = lat.plaquette()
+            trajs.append({"dH": float(st[2]), "accept": int(st[3]), "cg_iters": int(st[4]),
+                          "plaquette": list(plaq)})
+            say(f"fp32+surrogate: trajectory {len(trajs)} dH {st[2]:.5g} accept {int(st[3])} "
+                f"iters {int(st[4])} plaq {plaq[0]:.6f}")
+            done[0] = 0
+
+    for k in range(args.warmup):
+        one_step()
+        say(f"fp32+surrogate: warm-up step {k + 1}")
+    lat.prof_enable(args.prof_every)
+    ntraj0 = len(trajs)
+    t0 = sync_time()
+    for k in range(args.steps):
+        one_step()
+        say(f"fp32+surrogate: timed step {k + 1} t {sync_time() - t0:.1f} s")
+    t_timed = sync_time() - t0
+    prof = lat.prof_read(1)
+    lat.prof_enable(0)
+    ended = trajs[ntraj0:]
+    boundaries = len(ended)
+
+    # ---- figures
+    traj_per_hour_sg = 3600.0 * (args.steps / nstep) / t_timed
+    acc = float(np.mean([tr["accept"] for tr in trajs])) if trajs else None
+    value = traj_per_hour_sg * acc if acc is not None else 0.0
+
+    def per_hour(f):
+        s_traj = f["s_begin"] + nstep * float(np.mean(f["s_steps"]))
+        return s_traj, 3600.0 / s_traj
+
+    s32, tph32 = per_hour(f32)
+    s64, tph64 = per_hour(f64)
+    bytes32 = dslash_bytes(V, 8)
+    avg_ms = prof[0]
+    achieved = bytes32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    forms = {
+        "fp32_surrogate": {
+            "traj_per_hour": traj_per_hour_sg, "accepted_traj_per_hour": value,
+            "acceptance": acc, "trajectories_ended": len(trajs),
+            "trajectories": trajs,
+            "exp_minus_dH": finite_or_none(np.mean([np.exp(min(-tr["dH"], 700.0))
+                                                    for tr in trajs])) if trajs else None,
+            "s_per_traj": t_timed * nstep / args.steps,
+            "dslash_ms": avg_ms,
+            "training": {"pairs": npairs, "held_out": ntest, "adam_steps": args.sg_steps,
+                         "train_s": t_train,
+                         "train_loss_first_last": [float(np.mean(hist[:10])),
+                                                   float(np.mean(hist[-10:]))],
+                         "held_out_force_err2_untrained": before,
+                         "held_out_force_err2_trained": after}},
+        "fp32": {"traj_per_hour": tph32, "s_per_traj": s32, "s_begin": f32["s_begin"],
+                 "s_steps": f32["s_steps"], "dslash_ms": f32["dslash_ms"]},
+        "fp64": {"traj_per_hour": tph64, "s_per_traj": s64, "s_begin": f64["s_begin"],
+                 "s_steps": f64["s_steps"], "dslash_ms": f64["dslash_ms"]},
+    }
+    forms["traj_per_hour_ratio_fp32_surrogate_vs_fp64"] = traj_per_hour_sg / tph64
+    forms["traj_per_hour_ratio_fp32_vs_fp64"] = tph32 / tph64
+    out = {
+        "metric": "accepted traj/hour, U(4) 16^3x32: FP32+surrogate vs plain FP64; Dslash GB/s",
+        "value": value,
+        "unit": "accepted traj/hour",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_timed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (thermalised 8^3x16 links of this library's RHMC "
+                                "tiled to 16^3x32; generators seeded 1234)",
+        "config": {"workload": "4d_Q16 RHMC U(4) 16^3x32, lambda 2.0, bmass 0.4, G 0.05, "
+                               "kappa_u1 0, Nroot 2, Norder 15; trajectory length "
+                               f"{args.traj_length}, nstep {nstep}, nstep_gauge "
+                               f"{args.nstep_gauge}; FP32+surrogate form",
+                   "lattice": "16x16x16x32", "traj_length": args.traj_length,
+                   "nstep": nstep, "nstep_gauge": args.nstep_gauge,
+                   "step": "one outer MD step (gauge level, 2 x Nroot multi-shift solves, "
+                           "2 x Nroot fermion kicks); nstep steps = one trajectory, whose "
+                           "start and end fall inside the timed steps "
+                           f"({boundaries} in this run)",
+                   "start": "thermalised U(4) 8^3x16 configuration (same couplings) tiled "
+                            f"2x2x2x2 (plaquette {plaq_start[0]:.5f})",
+                   "form": "FP32 operator and CG with FP64 reliable updates; FP64 forces, "
+                           "actions and dH; learned surrogate force on the inner MD level"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(),
+                     "kernel": "k_dslash<float, false, 0>: D = fermion_op(sign +1), FP32",
+                     "launches": prof[1], "sampled": prof[2], "avg_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes32},
+        "forms": forms,
+        "dslash_isolated": {"fp64_ms": iso["fp64"],
+                            "fp64_GBs": dslash_bytes(V, 16) / iso["fp64"] / 1e6,
+                            "fp32_ms": iso["fp32"], "fp32_GBs": bytes32 / iso["fp32"] / 1e6},
+    }
+    if not args.no_cpu and trajs:
+        out["cpu_baseline"] = cpu_baseline(trajs[0]["cg_iters"], (2 * nstep + 1) * 2 * 15,
+                                           acc)
+    out["job_s"] = time.perf_counter() - t_job
+    print(json.dumps(out), flush=True)
+
+
+REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "susy_refbench_U4")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
+
+def cpu_baseline(iters_per_traj, force_poles, acceptance):
+    """The reference itself (its U(4) MPI build, oracle/_ref/susy_refbench_U4)
+    on the host's cores, timed on the full 16^3 x 32 lattice: one fermion_op,
+    3 iterations of the 15-shift congrad_multi and a 2-pole fermion_force;
+    a trajectory is priced at the GPU run's CG iteration count and force
+    poles.  Without that build (or without MPI), the C oracle on one core on
+    an 8^3 x 16 sub-volume, scaled by volume."""
+    import subprocess
+    ranks = min(16, os.cpu_count() or 1)
+    if os.path.exists(REF_BENCH) and os.path.exists(MPIEXEC):
+        cmd = [MPIEXEC, "-n", str(ranks), REF_BENCH, *map(str, DIMS), "1", "3", "2"]
+        say("cpu baseline: " + " ".join(cmd[2:]))
+        try:
+            p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                               timeout=240, cwd="/tmp")
+            line = [ln for ln in p.stdout.splitlines() if ln.startswith('{"ranks"')]
+            if p.returncode == 0 and line:
+                r = json.loads(line[-1])
+                t_traj = iters_per_traj * r["cg_iteration_s"] + force_poles * r["force_pole_s"]
+                return {"value": 3600.0 / t_traj * acceptance, "unit": "accepted traj/hour",
+                        "cores": r["ranks"], "kind": "reference",
+                        "sample": (f"reference U(4) MPI build on {r['ranks']} ranks, 16^3x32: "
+                                   f"fermion_op {r['fermion_op_s']:.3f} s, 15-shift CG "
+                                   f"iteration {r['cg_iteration_s']:.3f} s (3 timed), force "
+                                   f"pole {r['force_pole_s']:.3f} s (2 timed); priced at "
+                                   f"{iters_per_traj} CG iterations and {force_poles} force "
+                                   f"poles per trajectory (the GPU trajectory's counts), "
+                                   f"acceptance {acceptance:.2f} from the GPU run: est. "
+                                   f"{t_traj:.0f} s per trajectory"),
+                        "fermion_op_GBs": dslash_bytes(DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3], 16) / r["fermion_op_s"] / 1e9}
+            say("cpu baseline: reference run failed, using the oracle\n" + p.stdout[-2000:])
+        except (OSError, subprocess.SubprocessError, ValueError, KeyError) as e:
+            say(f"cpu baseline: reference run failed ({e}), using the oracle")
+    return cpu_baseline_port(iters_per_traj, force_poles, acceptance)
+
+
+def cpu_baseline_port(iters_per_traj, force_poles, acceptance):
     """The C oracle (a restatement of the reference, single core) timed on a
     sub-volume of the same workload: per-site costs of one 15-shift CG
     iteration and one fermion-force pole on U(4) 8^3 x 16, scaled to
-    16^3 x 32 and to the operation counts of the plain FP64 GPU trajectory."""
+    16^3 x 32 and to one trajectory's CG iterations and force poles."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
@@ -123,16 +378,14 @@ def cpu_baseline(fp64_stats, nforce_solves):
     t = time.perf_counter()
     O.fermion_force(0.01, ps[:1], O.rhmc()[1:2])
     t_pole = time.perf_counter() - t
-    scale = np.prod(DIMS) / np.prod(sub)
-    iters = np.mean([s[4] for s in fp64_stats])
-    t_traj = scale * (iters * t_cg1 + nforce_solves * 15 * t_pole)
-    acc = np.mean([s[3] for s in fp64_stats])
-    return {"value": 3600.0 / t_traj * acc, "unit": "accepted traj/hour", "cores": 1,
+    scale = float(np.prod(DIMS) / np.prod(sub))
+    t_traj = scale * (iters_per_traj * t_cg1 + force_poles * t_pole)
+    return {"value": 3600.0 / t_traj * acceptance, "unit": "accepted traj/hour", "cores": 1,
             "kind": "port",
             "sample": (f"C oracle, 1 core, U(4) 8^3x16 sub-volume: {t_cg1:.2f} s per 15-shift "
                        f"CG iteration, {t_pole:.2f} s per force pole; scaled x{scale:.0f} to "
-                       f"16^3x32 and to the FP64 GPU trajectory's {iters:.0f} CG iterations "
-                       f"and {nforce_solves}x15 force poles; acceptance {acc:.2f} from the GPU "
+                       f"16^3x32 and to {iters_per_traj} CG iterations and {force_poles} "
+                       f"force poles per trajectory, acceptance {acceptance:.2f} from the GPU "
                        f"run (est. {t_traj:.0f} s per trajectory)")}
 
 
@@ -153,141 +406,6 @@ def kernels_only(lat):
         out[name + "_ms"] = lat.time_fermion_op(h, h + 1, form=form, reps=10)
     print(json.dumps({"kernels": out, "copy_bytes_read": 16 * NCOL * NCOL * 16 * lat.V,
                       "copy_bytes_written": 16 * NCOL * NCOL * 16 * lat.V}), flush=True)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1, help="timed FP32-form trajectories")
-    ap.add_argument("--warmup", type=int, default=0, help="untimed FP32-form trajectories")
-    ap.add_argument("--fp64-steps", type=int, default=1, help="timed plain FP64 trajectories")
-    ap.add_argument("--traj-length", type=float, default=0.25)
-    ap.add_argument("--nstep", type=int, default=5)
-    ap.add_argument("--nstep-gauge", type=int, default=10)
-    ap.add_argument("--prof-every", type=int, default=8,
-                    help="time every n-th operator kernel launch with HIP events")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernels", action="store_true", help="operator launches only")
-    ap.add_argument("--sg-steps", type=int, default=300,
-                    help="Adam steps of the surrogate training (0: no surrogate form)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU decomposition is a design, not an implementation",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-
-    import susy_amd
-    lat = susy_amd.Lattice(NCOL, **workload(args.traj_length, args.nstep, args.nstep_gauge))
-    if args.kernels:
-        kernels_only(lat)
-        return
-    susy_amd.load(NCOL).susy_set_verbose(1)   # one progress line per solve (stderr)
-    V = lat.V
-    start = lat.get_links()
-    # warm-up trajectories move the start away from the cold configuration
-    for _ in range(args.warmup):
-        lat.update(susy_amd.FORM_FP32)
-    if args.warmup:
-        start = lat.get_links()
-
-    # isolated operator timing (HIP events, 20 launches each)
-    h = lat.tf_alloc(2)
-    rng = np.random.default_rng(5)
-    lat.tf_upload(h, (rng.standard_normal(lat.tf_shape())
-                      + 1j * rng.standard_normal(lat.tf_shape())) / np.sqrt(2))
-    iso = {nm: lat.time_fermion_op(h, h + 1, form=f, reps=20)
-           for nm, f in (("fp64", 0), ("fp32", 1))}
-
-    st32, t32, prof32, plaq32 = run_form(lat, susy_amd.FORM_FP32, args.steps, start,
-                                         args.prof_every)
-    if args.sg_steps > 0:
-        # the plain run records the (links, fermion force) pairs
-        lat.sg_init(seed=1, scale=1.0)
-        lat.sg_record(4 * (args.nstep + 2) * max(args.fp64_steps, 1))
-    st64, t64, prof64, plaq64 = run_form(lat, susy_amd.FORM_FP64, args.fp64_steps, start,
-                                         args.prof_every)
-    sg = None
-    if args.sg_steps > 0:
-        lat.sg_record(0)
-        npairs = lat.sg_pairs()
-        ntest = 2 if npairs > 4 else 0
-        held = list(range(npairs - ntest, npairs))
-        before = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
-        t = time.perf_counter()
-        hist = lat.sg_train(0, npairs - ntest, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
-        t_train = time.perf_counter() - t
-        after = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
-        stsg, tsg, profsg, plaqsg = run_form(lat, susy_amd.FORM_FP32 | susy_amd.FORM_SURROGATE,
-                                             args.steps, start, args.prof_every)
-        sg = (stsg, tsg, profsg, plaqsg, {
-            "pairs": npairs, "held_out": ntest, "train_steps": args.sg_steps,
-            "train_s": t_train, "train_loss_first_last": [float(np.mean(hist[:10])),
-                                                          float(np.mean(hist[-10:]))],
-            "held_out_force_err2_untrained": before, "held_out_force_err2_trained": after})
-
-    def summary(stats, secs, prof, plaq, nbytes):
-        acc = sum(s[3] for s in stats)
-        return {"traj_per_hour": 3600.0 * len(stats) / secs,
-                "accepted_traj_per_hour": 3600.0 * acc / secs,
-                "acceptance": acc / len(stats),
-                "dH": [float(s[2]) for s in stats],
-                # null when a cold-start transient (dH << 0) overflows exp(-dH)
-                "exp_minus_dH": finite_or_none(np.mean([np.exp(min(-s[2], 700.0))
-                                                        for s in stats])),
-                "cg_iters": [int(s[4]) for s in stats],
-                "s_per_traj": secs / len(stats),
-                "plaquette_end": list(plaq),
-                "dslash_ms": prof[0],
-                "dslash_GBs": dslash_bytes(V, nbytes) / (prof[0] * 1e-3) / 1e9 if prof[0] else 0}
-
-    f32 = summary(st32, t32, prof32, plaq32, 8)
-    f64 = summary(st64, t64, prof64, plaq64, 16)
-    forms = {"fp32": f32, "fp64": f64,
-             "speedup_fp32_vs_fp64": f64["s_per_traj"] / f32["s_per_traj"],
-             "dH_abs_diff_first_traj": abs(st32[0][2] - st64[0][2])}
-    if sg is not None:
-        fsg = summary(sg[0], sg[1], sg[2], sg[3], 8)
-        fsg["training"] = sg[4]
-        forms["fp32_surrogate"] = fsg
-        forms["speedup_fp32_surrogate_vs_fp64"] = f64["s_per_traj"] / fsg["s_per_traj"]
-    nforce_solves = args.nstep * 2 + 1
-    bytes32 = dslash_bytes(V, 8)
-    avg_ms = prof32[0]
-    achieved = bytes32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    out = {
-        "metric": "accepted traj/hour, U(4) 16^3x32: FP32+surrogate vs plain FP64; Dslash GB/s",
-        "value": f32["accepted_traj_per_hour"],
-        "unit": "accepted traj/hour",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * t32 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (cold start, generators seeded 1234)",
-        "config": {"workload": "4d_Q16 RHMC U(4) 16^3x32, lambda 2.0, bmass 0.4, G 0.05, "
-                               "kappa_u1 0, Nroot 2, Norder 15; FP32 form from a cold start",
-                   "lattice": "16x16x16x32", "traj_length": args.traj_length,
-                   "nstep": args.nstep, "nstep_gauge": args.nstep_gauge,
-                   "form": "FP32 operator and CG with FP64 reliable updates; FP64 forces, "
-                           "actions and dH",
-                   "surrogate": "value is the FP32 form; the FP32+surrogate form (trained "
-                                "in this run on the FP64 trajectory's force pairs) is "
-                                "reported under forms.fp32_surrogate"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(),
-                     "kernel": "k_dslash<float, false, 0>: D = fermion_op(sign +1), FP32",
-                     "launches": prof32[1], "sampled": prof32[2], "avg_ms": avg_ms,
-                     "algorithmic_bytes_per_launch": bytes32},
-        "forms": forms,
-        "dslash_isolated": {"fp64_ms": iso["fp64"],
-                            "fp64_GBs": dslash_bytes(V, 16) / iso["fp64"] / 1e6,
-                            "fp32_ms": iso["fp32"], "fp32_GBs": bytes32 / iso["fp32"] / 1e6},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(st64, 2 * nforce_solves)
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -34,7 +34,8 @@ LIB_SRC  := cmplx cexp clog csqrt ce_itheta dcmplx dcexp dclog dcsqrt \
 OBJS := $(SUSY_SRC:%=$(OBJ)/susy_%.o) $(GEN_SRC:%=$(OBJ)/gen_%.o) \
         $(LIB_SRC:%=$(OBJ)/lib_%.o)
 
-all: $(OUT)/libsusyref_U2.so $(OUT)/susy_hmc_U2
+all: $(OUT)/libsusyref_U2.so $(OUT)/susy_hmc_U2 $(OUT)/susy_refbench_U4 \
+     $(OUT)/susy_hmc_U2_hip
 
 $(OBJ):
 	mkdir -p $(OBJ)
@@ -55,6 +56,68 @@ $(OUT)/libsusyref_U2.so: $(OBJS) $(OBJ)/ref_harness.o
 
 $(OUT)/susy_hmc_U2: $(OBJS) $(OBJ)/control.o
 	$(CC) -o $@ $^ $(LIBS)
+
+# ---------------------------------------------------------------------------
+# The drop-in test: the reference's susy_hmc with its own definitions of the
+# five replaced symbols localised (objcopy -L) and libsusy_hip_U2.so linked
+# in their place; oracle/hip_bind.c registers the driver's globals through
+# --wrap of readin and compute_Fmunu.  The library is found through a run
+# path relative to the binary, so the pair can move together; MKL's directory
+# is a run path of the binary only (not of the HIP runtime, which needs the
+# system's C++ library).
+HIPLIB  := susy_amd/lib
+DROPIN  := fermion_op DSq congrad_multi fermion_force gauge_force
+LOCALISED := utilities congrad_multi update_h
+$(OBJ)/susy_%.local.o: $(OBJ)/susy_%.o
+	objcopy $(DROPIN:%=-L %) $< $@
+$(OBJ)/hip_bind.o: oracle/hip_bind.c | $(OBJ)
+	$(CC) $(CFLAGS) -c $< -o $@
+OBJS_DROPIN := $(filter-out $(LOCALISED:%=$(OBJ)/susy_%.o),$(OBJS)) \
+               $(LOCALISED:%=$(OBJ)/susy_%.local.o)
+$(OUT)/susy_hmc_U2_hip: $(OBJS_DROPIN) $(OBJ)/control.o $(OBJ)/hip_bind.o $(HIPLIB)/libsusy_hip_U2.so
+	$(CC) -o $@ $(OBJS_DROPIN) $(OBJ)/control.o $(OBJ)/hip_bind.o \
+	  -Wl,--wrap=readin -Wl,--wrap=compute_Fmunu \
+	  -L$(HIPLIB) -lsusy_hip_U2 $(MKLDIR)/libmkl_rt.so -lm \
+	  -Wl,-rpath-link,/usr/lib/x86_64-linux-gnu -Wl,--enable-new-dtags \
+	  '-Wl,-rpath,$$ORIGIN/../../susy_amd/lib:$(MKLDIR)'
+
+# ---------------------------------------------------------------------------
+# The CPU baseline: the reference at U(4) as an MPI program (Make_mpi's
+# com_mpi.c), with oracle/ref_bench.c as its main.  The reference selects the
+# gauge group by the NCOL define of include/susy.h:16 and every source reaches
+# that header as "../include/susy.h"; the build generates a copy of the header
+# with NCOL 4 under $(OUT4)/hdr/include and compiles with -I- so that quoted
+# includes are searched in $(OUT4)/hdr/susy first (where ../include/susy.h is
+# the generated copy) and then in the reference's own directories.
+MPICC  ?= /opt/conda/bin/mpicc
+OUT4   := $(OUT)/u4mpi
+OBJ4   := $(OUT4)/obj
+HDR4   := $(OUT4)/hdr
+CFLAGS4 := -O3 -fPIC -fno-strict-aliasing -w -DMPI -DPRECISION=2 \
+           -D_FILE_OFFSET_BITS=64 -D_LARGEFILE_SOURCE64 -DPHI_ALGORITHM -DHMC_ALGORITHM \
+           -I$(HDR4)/susy -I- -I$(REF)/susy -I$(REF)/generic
+GEN4_SRC := $(filter-out com_vanilla,$(GEN_SRC)) com_mpi
+OBJS4 := $(SUSY_SRC:%=$(OBJ4)/susy_%.o) $(GEN4_SRC:%=$(OBJ4)/gen_%.o) \
+         $(LIB_SRC:%=$(OBJ4)/lib_%.o)
+
+$(HDR4)/include/susy.h: $(REF)/include/susy.h
+	mkdir -p $(HDR4)/include $(HDR4)/susy $(OBJ4)
+	sed 's/^#define NCOL 2$$/#define NCOL 4/' $< > $@
+	grep -q '^#define NCOL 4$$' $@
+
+$(OBJ4)/susy_%.o: $(REF)/susy/%.c $(HDR4)/include/susy.h
+	MPICH_CC=$(CC) $(MPICC) $(CFLAGS4) -c $< -o $@
+$(OBJ4)/gen_%.o: $(REF)/generic/%.c $(HDR4)/include/susy.h
+	MPICH_CC=$(CC) $(MPICC) $(CFLAGS4) -c $< -o $@
+$(OBJ4)/lib_%.o: $(REF)/libraries/%.c $(HDR4)/include/susy.h
+	MPICH_CC=$(CC) $(MPICC) $(CFLAGS4) -c $< -o $@
+$(OBJ4)/ref_harness.o: oracle/ref_harness.c $(HDR4)/include/susy.h
+	MPICH_CC=$(CC) $(MPICC) $(CFLAGS4) -c $< -o $@
+$(OBJ4)/ref_bench.o: oracle/ref_bench.c $(HDR4)/include/susy.h
+	MPICH_CC=$(CC) $(MPICC) $(CFLAGS4) -c $< -o $@
+
+$(OUT)/susy_refbench_U4: $(OBJS4) $(OBJ4)/ref_harness.o $(OBJ4)/ref_bench.o
+	MPICH_CC=$(CC) $(MPICC) -o $@ $^ $(LIBS)
 
 clean:
 	rm -rf $(OUT)

@@ -41,6 +41,12 @@ int dslash_init_tables(const PlaqTerms &pt) {
 #ifndef DS_WAVES
 #define DS_WAVES (NCOL >= 5 ? 1 : 2)
 #endif
+#ifndef DS_WAVES64
+#define DS_WAVES64 DS_WAVES
+#endif
+#ifndef DS_WAVES32
+#define DS_WAVES32 DS_WAVES
+#endif
 // dynamic LDS bytes requested per block: caps the blocks resident per CU
 // (160 KB of LDS per CU), and with it the sites in flight per XCD, whose
 // neighbourhood must fit the XCD's L2.  0 leaves occupancy to registers.
@@ -184,8 +190,8 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
 }
 
 template <typename R> struct DsSites;
-template <> struct DsSites<double> { static constexpr int n = DS_SITES64; };
-template <> struct DsSites<float> { static constexpr int n = DS_SITES32; };
+template <> struct DsSites<double> { static constexpr int n = DS_SITES64, waves = DS_WAVES64; };
+template <> struct DsSites<float> { static constexpr int n = DS_SITES32, waves = DS_WAVES32; };
 
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
@@ -201,7 +207,7 @@ template <> struct DsSites<float> { static constexpr int n = DS_SITES32; };
 // occupancy, so their neighbourhoods compete less for the XCD's L2.
 template <typename R, bool DAG, int EPI>
 __global__ void __launch_bounds__(DsSites<R>::n * DS_SPLIT)
-    __attribute__((amdgpu_waves_per_eu(DS_WAVES)))
+    __attribute__((amdgpu_waves_per_eu(DsSites<R>::waves)))
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
          const typename CT<R>::t *__restrict__ pdet, const typename CT<R>::t *__restrict__ aux,
@@ -624,3 +630,4 @@ int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float s
   return dslash_launch<float>(c, tmp, dst, -1, src, shift, partial, nblocks, c->f_links,
                               c->f_uinv, c->f_pdet, c->f_aux2, skip, nullptr, false);
 }
+

@@ -20,6 +20,12 @@ acceptance, <exp(-dH)>, <|dH|>, CG iterations and seconds per trajectory.
 
     python -m susy_amd.surrogate_run [--train-traj 200] [--pairs 2000]
         [--adam 5000] [--traj 50] [--nsteps 20] [--sg-nsteps 20]
+        [--save-params THETA.npy] [--save-start START.milc]
+
+--save-params / --save-start write the trained parameters (float32, the
+order of susy_sg_get_params) and the final configuration of the generating
+chain (MILC file), the inputs of the test that measures what the trained
+network gains over the plain form.
 """
 import argparse
 import json
@@ -68,6 +74,8 @@ def main(argv=None):
     ap.add_argument("--nsteps", default="20", help="outer steps of the plain forms (comma list)")
     ap.add_argument("--sg-nsteps", default="20", help="outer steps of the surrogate form")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--save-params", default=None)
+    ap.add_argument("--save-start", default=None)
     a = ap.parse_args(argv)
 
     gen = susy_amd.Lattice(2, iseed=a.seed, nstep=10, **COUPLINGS)
@@ -95,6 +103,10 @@ def main(argv=None):
         f"{after:.4f}; {a.adam} Adam steps in {t_train:.1f} s")
     theta = gen.sg_get_params()
     start = gen.get_links()
+    if a.save_params:
+        np.save(a.save_params, theta.astype(np.float32))
+    if a.save_start:
+        gen.save_config(a.save_start)
     gen.close()
 
     plain = [int(x) for x in a.nsteps.split(",") if x]
