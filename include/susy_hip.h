@@ -171,6 +171,18 @@ int susy_stream_copy_dev(susy_ctx *c, int h_src, int h_dst);
 int susy_congrad_multi_dev(susy_ctx *c, int h_src, int h_psim, int nshift,
                            const double *shifts, int MaxCG, double errormin,
                            double *size_r, int form);
+// The operator without allocation: dst = D src (sign +1) or D^dagger src
+// (sign -1) in FP64, like susy_fermion_op_dev, for src and dst given as device
+// pointers to Twist_Fermion fields in the context's layout (16 NCOL^2 planes
+// of V complex doubles, element (i,j) of component k at site s at complex
+// index (k NCOL^2 + i NCOL + j) V + s).  No counterpart in the reference: it
+// is for callers that cannot allocate (code under HIP graph capture, serving
+// loops).  It calls no hipMalloc/hipFree: its scratch (6 complex doubles per
+// site) is a fixed 16 MiB device array of the library, so V <= 174762
+// (larger lattices return SUSY_E_ARG).  All calls on one GPU share that
+// array: calls that overlap on one GPU race.  Synchronises the context's
+// stream before returning.
+int susy_fermion_op_noalloc(susy_ctx *c, const double *src, double *dst, int sign);
 // Fermion force from nsol solutions; updates the device momenta.
 int susy_fermion_force_dev(susy_ctx *c, double eps, int h_sol, int nsol,
                            const double *amps, double *norm);

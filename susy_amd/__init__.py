@@ -56,7 +56,7 @@ EXPORTS = [
     "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
     "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
-    "susy_traj_begin", "susy_traj_step", "susy_traj_end",
+    "susy_traj_begin", "susy_traj_step", "susy_traj_end", "susy_fermion_op_noalloc",
 ]
 
 _libs = {}
@@ -96,6 +96,7 @@ def load(ncol):
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
         "susy_traj_begin": (i, [vp, i]), "susy_traj_step": (i, [vp]),
         "susy_traj_end": (i, [vp, dp]),
+        "susy_fermion_op_noalloc": (i, [vp, vp, vp, i]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
         "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
         "susy_set_site_order": (i, [vp, i]),

@@ -172,6 +172,7 @@ static int ctx_setup(susy_ctx *c) {
     g_tables_ready = true;
   }
   HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  RET(noalloc_init_device(c));
   const size_t mat5 = (size_t)NUMLINK * NN * V, tfn = (size_t)NTF * NN * V;
   RET(dalloc(&c->d_nbr, (size_t)NGATHER * V));
   RET(dalloc(&c->d_t, (size_t)V));

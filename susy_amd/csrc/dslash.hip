@@ -27,6 +27,8 @@
 // same L2.
 #include "susy_internal.h"
 
+#include <mutex>
+
 __constant__ PlaqTerms c_pt;
 
 int dslash_init_tables(const PlaqTerms &pt) {
@@ -496,7 +498,7 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
                          const typename CT<R>::t *U, const typename CT<R>::t *Uinv,
                          const typename CT<R>::t *pdet, typename CT<R>::t *aux,
                          const int *skip, typename CT<R>::t *aux_out = nullptr,
-                         bool prep = true) {
+                         bool prep = true, bool sampled = true) {
   if (sign != 1 && sign != -1) {
     set_error("fermion_op: incorrect sign %d", sign);
     return SUSY_E_ARG;
@@ -529,7 +531,7 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
   // sampled timing covers the plain operator D (sign +1, no epilogue) only
   const int pi = sizeof(R) == sizeof(float) ? 1 : 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->prof_every > 0 && sign == 1 && p0 == nullptr &&
+  if (sampled && c->prof_every > 0 && sign == 1 && p0 == nullptr &&
       (c->prof_launch[pi]++ % c->prof_every) == 0) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -631,3 +633,70 @@ int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float s
                               c->f_uinv, c->f_pdet, c->f_aux2, skip, nullptr, false);
 }
 
+// ---------------------------------------------------------------------
+// Operator without allocation (susy_fermion_op_noalloc).  Its only scratch
+// memory -- the G-term pre-pass values, 6 complex numbers per site -- lives
+// in this fixed device array instead of the context's buffer.  Every device
+// the library runs on holds one copy until the process ends.  The pre-pass
+// writes every value the operator then reads, so nothing in it needs
+// clearing between calls; calls that overlap on one device race on it.
+#define NOALLOC_SCRATCH_BYTES (16u << 20)
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+// device address of `scratch` on each device (the name itself is only a
+// host-side handle of the symbol)
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Called by susy_create with the context's device current: looks up that
+// device's copy of the array once.
+int noalloc_init_device(susy_ctx *c) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("device %d beyond the %d the library tracks", dev, NOALLOC_MAX_DEVICES);
+    return SUSY_E_STATE;
+  }
+  c->dev = dev;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return SUSY_E_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  return 0;
+}
+
+extern "C" int susy_fermion_op_noalloc(susy_ctx *c, const double *src, double *dst, int sign) {
+  if (!c) return SUSY_E_STATE;
+  if (!src || !dst) {
+    set_error("susy_fermion_op_noalloc: null field pointer");
+    return SUSY_E_ARG;
+  }
+  if (sign != 1 && sign != -1) {
+    set_error("susy_fermion_op_noalloc: incorrect sign %d", sign);
+    return SUSY_E_ARG;
+  }
+  const size_t need = (size_t)6 * c->V * sizeof(double2);
+  if (need > NOALLOC_SCRATCH_BYTES) {
+    set_error("susy_fermion_op_noalloc: %d sites need %zu bytes of scratch; the fixed "
+              "array holds %u (at most %u sites)", c->V, need, NOALLOC_SCRATCH_BYTES,
+              (unsigned)(NOALLOC_SCRATCH_BYTES / (6 * sizeof(double2))));
+    return SUSY_E_ARG;
+  }
+  void *aux = g_scratch_dev[c->dev];
+  if (!aux) {
+    set_error("susy_fermion_op_noalloc: scratch of device %d not set up", c->dev);
+    return SUSY_E_STATE;
+  }
+  int r = dslash_launch<double>(c, (const double2 *)src, (double2 *)dst, sign, nullptr, 0.0,
+                                nullptr, nullptr, c->d_links, c->d_uinv, c->d_pdet,
+                                (double2 *)aux, nullptr, nullptr, true, false);
+  if (r) return r;
+  HIPCHK(hipStreamSynchronize(c->s));
+  return 0;
+}

@@ -212,6 +212,7 @@ struct susy_ctx {
   std::vector<double> lambda_gen;   // Lambda generators, DIMF * NN * 2
 
   hipStream_t s = nullptr;
+  int dev = 0;                      // HIP device the context was created on
   int *d_nbr = nullptr;             // NGATHER * V
   int *d_t = nullptr;               // t coordinate per site
   // gauge fields (FP64, SoA) and derived fields
@@ -294,6 +295,8 @@ int dsq_pair64(susy_ctx *c, const double2 *src, double2 *tmp, double2 *dst, doub
                double *partial, int *nblocks, const int *skip = nullptr, bool prep_src = true);
 int dsq_pair32(susy_ctx *c, const float2 *src, float2 *tmp, float2 *dst, float shift,
                double *partial, int *nblocks, const int *skip = nullptr, bool prep_src = true);
+
+int noalloc_init_device(susy_ctx *c);   // (dslash.hip) fixed scratch of this device
 
 int fields_recompute(susy_ctx *c);      // det, Uinv, plaqdet, ZW*, DmuUmu, Fmunu
 int fields_refresh32(susy_ctx *c);      // FP32 copies of links, Uinv, plaqdet
