@@ -2,28 +2,36 @@
 
 Workload: U(4) on a 16^3 x 32 lattice, lambda = 2.0, bmass = 0.4, G = 0.05,
 kappa_u1 = 0, fmass = 0, Nroot = 2, Norder = 15, antiperiodic temporal
-fermion BC, CG stopping at |r|^2 <= 1e-10 |src|^2 (error_per_site 1e-5).
-Trajectories of length 1 with nstep = 20 outer (fermion) steps and
-nstep_gauge = 10 gauge steps inside each half step (MD step 0.05).
+fermion BC, CG stopping at |r|^2 <= 1e-10 |src|^2 (error_per_site 1e-5),
+MD step 0.05 on the fermion level (the step of the unit-length, nstep-20
+trajectories of the 12^3 x 24 configuration) with 10 gauge steps inside each
+half step.  Trajectories are 0.25 long (nstep 5): at this volume a
+multi-shift solve takes 4000-6000 CG iterations (6-9 s in FP32), so a unit
+trajectory (84 solves) takes 10-15 minutes and cannot end inside a
+benchmark run; a 0.25 trajectory (24 solves) can.  Every per-hour figure is
+counted in unit-length equivalents (a trajectory of length 0.25 counts 0.25),
+i.e. it is MD time per hour, which does not depend on the trajectory length
+chosen.
 
-Start: a thermalised configuration.  A 16^3 x 32 lattice needs tens of unit
+Start: a thermalised configuration.  A 16^3 x 32 lattice needs many
 trajectories from a cold start before its actions and solver costs settle,
 far more than a benchmark run can spend, so the start is an 8^3 x 16
 configuration with the same couplings thermalised by this library's own RHMC
-(committed as susy_amd/configs/therm_u4_8x8x8x16.milc; made by
-`python -m susy_amd.start_config`) tiled 2 x 2 x 2 x 2 onto 16^3 x 32.
+(committed as susy_amd/configs/therm_u4_8x8x8x16.milc with its history in the
+.info file beside it; made by `python -m susy_amd.start_config`) tiled
+2 x 2 x 2 x 2 onto 16^3 x 32.
 Momenta and pseudofermions come from the reference's generators seeded with
 1234.  Every form starts from the same links with the same seed, so their
 first trajectories have identical momenta and pseudofermions.
 
-One step = one outer MD step of the trajectory (update_o.c:80-113): the gauge
-level (2 x nstep_gauge Omelyan gauge steps), 2 x Nroot multi-shift solves and
-2 x Nroot fermion-force kicks.  A trajectory is nstep = 20 such steps plus its
-start (heatbath: 2 x Nroot solves; first kick) and end (action, Metropolis),
-so `--steps 20` times exactly one trajectory's worth of work: the timed
-steps run on from the warm-up steps of the first trajectory into the next,
-and the end of one trajectory and the start of the next fall inside the
-timed region.
+One step = one multi-shift solve of one root and its fermion-force kick (the
+first of each half outer step also runs the gauge level before it): the
+unit of the hot path, congrad_multi + fermion_force (update_o.c:80-113).  A
+trajectory is its start (heatbath: 2 x Nroot solves; first kick), nstep x
+2 x Nroot = 20 steps, and its end (action, Metropolis test), so `--steps 20`
+times exactly one trajectory's work: the timed steps run on from the warm-up
+steps of the first trajectory into the next, and the end of one trajectory
+and the start of the next fall inside the timed region.
 
 Forms:
   * FP32 + surrogate (headline `value`): FP32 operator and multi-shift CG with
@@ -32,18 +40,28 @@ Forms:
     then `--steps` timed steps, trajectories completed and Metropolis-tested
     as they end.
   * FP32 (no surrogate) and plain FP64: the start of one trajectory and
-    `--fp32-steps` / `--fp64-steps` timed outer steps from the same links and
-    seed; their trajectory time is start + nstep x the mean step time.
-The surrogate is trained in the run itself, on the (links, fermion force)
-pairs recorded at every outer kick of the FP32 and FP64 legs (2 held out),
-with the library's own forward and backward kernels (Adam, `--sg-steps`).
+    `--fp32-steps` / `--fp64-steps` timed steps from the same links and
+    seed; their trajectory time is start + 2 Nroot nstep x the mean step
+    time.
+The surrogate's parameters come from `python -m susy_amd.surrogate_train`
+(committed as susy_amd/configs/surrogate_u4_theta.npy): the library's own
+forward and backward kernels trained it (Adam) on (links, fermion force)
+pairs of the plain RHMC on the 8^3 x 16 lattice of the same couplings -- the
+network is local, so its parameters carry over to 16^3 x 32.  The run
+reports its force error on the pairs the FP32 leg records at this size
+(pairs it was not trained on), and can fine-tune on them (`--sg-steps`).
 
-value = accepted trajectories per hour of the FP32+surrogate form =
-3600 x acceptance x (timed steps / nstep) / timed seconds, with the
-acceptance of the trajectories that ended in the run (reported with their
-count and dH).  The ratio to the plain FP64 form's trajectories per hour is
-reported beside it; acceptance rates from longer runs of all three forms are
-not part of this command.
+value = accepted trajectories per hour of the FP32+surrogate form, in
+unit-length equivalents = 3600 x acceptance x traj_length x (timed steps /
+steps per trajectory) / timed seconds.  The acceptance is
+the mean Metropolis probability min(1, exp(-dH)) of the trajectories that
+ended in the run (its expectation is the acceptance rate, with far less
+variance than the 0/1 outcomes of the one or two trajectories a run of this
+length ends; the outcomes and dH are reported beside it).  The plain FP64
+and FP32 forms integrate the same Hamiltonian with the same integrator (their
+dH on identical inputs agree to ~1e-4), so they share an acceptance rate and
+are compared by trajectories per hour; the ratios to the plain FP64 form are
+reported.
 
 `--kernels` runs only a short operator workload (5 field copies for the
 memory-counter calibration, then 10 launches of the operator in each
@@ -69,11 +87,13 @@ WORKLOAD = "u4_16x16x16x32"
 # counter summary of the operator kernel at this workload (HBM bytes per launch)
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"k_dslash_fp32_{WORKLOAD}_counters.txt")
 SEED = 1234
+# surrogate parameters trained by susy_amd/surrogate_train.py
+SG_THETA = os.path.join(ROOT, "susy_amd", "configs", "surrogate_u4_theta.npy")
 
 
 def workload(traj_length, nstep, nstep_gauge):
     return dict(dims=DIMS, PBC=-1, iseed=SEED, Nroot=2, Norder=15, lam=2.0, kappa_u1=0.0,
-                bmass=0.4, fmass=0.0, G=0.05, niter=5000, error_per_site=1e-5,
+                bmass=0.4, fmass=0.0, G=0.05, niter=20000, error_per_site=1e-5,
                 traj_length=traj_length, nstep=nstep, nstep_gauge=nstep_gauge)
 
 
@@ -119,7 +139,7 @@ def partial_form(lat, susy_amd, form, name, start, nsteps, prof_every):
     steps = []
     for k in range(nsteps):
         t = time.perf_counter()
-        lat.traj_step()
+        lat.traj_substep()
         steps.append(time.perf_counter() - t)
         say(f"{name}: step {k + 1} {steps[-1]:.2f} s")
     prof = lat.prof_read(form & 1)
@@ -131,18 +151,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20,
-                    help="timed outer MD steps of the FP32+surrogate form")
+                    help="timed steps (one root's solve + kick) of the FP32+surrogate form")
     ap.add_argument("--warmup", type=int, default=5, help="untimed outer steps before them")
-    ap.add_argument("--fp32-steps", type=int, default=2, help="timed steps, FP32 form")
-    ap.add_argument("--fp64-steps", type=int, default=2, help="timed steps, plain FP64 form")
-    ap.add_argument("--traj-length", type=float, default=1.0)
-    ap.add_argument("--nstep", type=int, default=20)
+    ap.add_argument("--fp32-steps", type=int, default=1, help="timed steps, FP32 form")
+    ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps, plain FP64 form")
+    ap.add_argument("--traj-length", type=float, default=0.25)
+    ap.add_argument("--nstep", type=int, default=5)
     ap.add_argument("--nstep-gauge", type=int, default=10)
     ap.add_argument("--prof-every", type=int, default=8,
                     help="time every n-th operator kernel launch with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="operator launches only")
-    ap.add_argument("--sg-steps", type=int, default=200, help="Adam steps of the surrogate")
+    ap.add_argument("--sg-steps", type=int, default=0,
+                    help="Adam steps fine-tuning the surrogate on this run's force pairs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -162,6 +183,8 @@ def main():
         return
     susy_amd.load(NCOL).susy_set_verbose(1)   # one progress line per solve (stderr)
     V, nstep = lat.V, args.nstep
+    per_traj = 2 * 2 * nstep          # steps per trajectory: 2 halves x Nroot roots x nstep
+    ell = args.traj_length
 
     # ---- start configuration: thermalised 8^3 x 16 links tiled onto 16^3 x 32
     from susy_amd import start_config
@@ -179,27 +202,29 @@ def main():
            for nm, f in (("fp64", 0), ("fp32", 1))}
     say(f"isolated D: FP64 {iso['fp64']:.3f} ms, FP32 {iso['fp32']:.3f} ms")
 
-    # ---- FP32 and FP64 forms: start + a few steps each, recording force pairs
-    npair_cap = 2 * (args.fp32_steps + args.fp64_steps) + 2
+    # ---- FP32 and FP64 forms: start + a few steps each; the FP32 leg records
+    # its (links, fermion force) pairs, on which the surrogate is evaluated
+    npair_cap = 1 + args.fp32_steps
     lat.sg_init(seed=1, scale=1.0)
+    theta = np.load(SG_THETA)
+    lat.sg_set_params(theta)
     lat.sg_record(npair_cap)
     f32 = partial_form(lat, susy_amd, susy_amd.FORM_FP32, "fp32", start, args.fp32_steps,
                        args.prof_every)
+    lat.sg_record(0)
     f64 = partial_form(lat, susy_amd, susy_amd.FORM_FP64, "fp64", start, args.fp64_steps,
                        args.prof_every)
-    lat.sg_record(0)
 
-    # ---- surrogate training on the recorded pairs
+    # ---- the surrogate (trained offline on the 8^3 x 16 chain of the same
+    # couplings): force error on this lattice's pairs, optional fine-tuning
     npairs = lat.sg_pairs()
-    ntest = 2 if npairs > 4 else 0
-    held = list(range(npairs - ntest, npairs))
-    before = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
+    err_pairs = [lat.sg_pair_loss(i) for i in range(npairs)]
     t = time.perf_counter()
-    hist = lat.sg_train(0, npairs - ntest, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
+    if args.sg_steps > 0:
+        lat.sg_train(0, npairs, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
     t_train = time.perf_counter() - t
-    after = float(np.mean([lat.sg_pair_loss(i) for i in held])) if held else None
-    say(f"surrogate: {npairs} pairs, {args.sg_steps} Adam steps {t_train:.1f} s, held-out "
-        f"force error^2 {before} -> {after}")
+    say(f"surrogate: relative force error^2 on this run's {npairs} pairs "
+        f"{float(np.mean(err_pairs)):.4f}; {args.sg_steps} fine-tuning steps")
 
     # ---- headline: FP32 + surrogate, warmup steps then timed steps
     form = susy_amd.FORM_FP32 | susy_amd.FORM_SURROGATE
@@ -211,9 +236,9 @@ def main():
     def one_step():
         if done[0] == 0:
             lat.traj_begin(form)
-        lat.traj_step()
+        lat.traj_substep()
         done[0] += 1
-        if done[0] == nstep:
+        if done[0] == per_traj:
             st = lat.traj_end()
             plaq = lat.plaquette()
             trajs.append({"dH": float(st[2]), "accept": int(st[3]), "cg_iters": int(st[4]),
@@ -238,13 +263,15 @@ def main():
     boundaries = len(ended)
 
     # ---- figures
-    traj_per_hour_sg = 3600.0 * (args.steps / nstep) / t_timed
-    acc = float(np.mean([tr["accept"] for tr in trajs])) if trajs else None
+    traj_per_hour_sg = 3600.0 * ell * (args.steps / per_traj) / t_timed
+    # acceptance: mean Metropolis probability of the ended trajectories
+    acc = float(np.mean([min(1.0, np.exp(min(-tr["dH"], 700.0))) for tr in trajs])) \
+        if trajs else None
     value = traj_per_hour_sg * acc if acc is not None else 0.0
 
     def per_hour(f):
-        s_traj = f["s_begin"] + nstep * float(np.mean(f["s_steps"]))
-        return s_traj, 3600.0 / s_traj
+        s_traj = f["s_begin"] + per_traj * float(np.mean(f["s_steps"]))
+        return s_traj, 3600.0 * ell / s_traj
 
     s32, tph32 = per_hour(f32)
     s64, tph64 = per_hour(f64)
@@ -254,18 +281,20 @@ def main():
     forms = {
         "fp32_surrogate": {
             "traj_per_hour": traj_per_hour_sg, "accepted_traj_per_hour": value,
-            "acceptance": acc, "trajectories_ended": len(trajs),
+            "acceptance": acc,
+            "acceptance_estimator": "mean of min(1, exp(-dH)) over the ended trajectories",
+            "accepted_outcomes": [tr["accept"] for tr in trajs],
+            "trajectories_ended": len(trajs),
             "trajectories": trajs,
             "exp_minus_dH": finite_or_none(np.mean([np.exp(min(-tr["dH"], 700.0))
                                                     for tr in trajs])) if trajs else None,
-            "s_per_traj": t_timed * nstep / args.steps,
+            "s_per_traj": t_timed * per_traj / args.steps,
             "dslash_ms": avg_ms,
-            "training": {"pairs": npairs, "held_out": ntest, "adam_steps": args.sg_steps,
-                         "train_s": t_train,
-                         "train_loss_first_last": [float(np.mean(hist[:10])),
-                                                   float(np.mean(hist[-10:]))],
-                         "held_out_force_err2_untrained": before,
-                         "held_out_force_err2_trained": after}},
+            "surrogate": {"parameters": os.path.relpath(SG_THETA, ROOT),
+                          "trained": "offline, susy_amd/surrogate_train.py on the 8^3x16 "
+                                     "chain of the same couplings",
+                          "force_err2_on_run_pairs": [float(x) for x in err_pairs],
+                          "fine_tuning_steps": args.sg_steps, "fine_tuning_s": t_train}},
         "fp32": {"traj_per_hour": tph32, "s_per_traj": s32, "s_begin": f32["s_begin"],
                  "s_steps": f32["s_steps"], "dslash_ms": f32["dslash_ms"]},
         "fp64": {"traj_per_hour": tph64, "s_per_traj": s64, "s_begin": f64["s_begin"],
@@ -283,15 +312,17 @@ def main():
         "dtype": "f32", "data": "synthetic (thermalised 8^3x16 links of this library's RHMC "
                                 "tiled to 16^3x32; generators seeded 1234)",
         "config": {"workload": "4d_Q16 RHMC U(4) 16^3x32, lambda 2.0, bmass 0.4, G 0.05, "
-                               "kappa_u1 0, Nroot 2, Norder 15; trajectory length "
-                               f"{args.traj_length}, nstep {nstep}, nstep_gauge "
-                               f"{args.nstep_gauge}; FP32+surrogate form",
+                               "kappa_u1 0, Nroot 2, Norder 15; MD step "
+                               f"{ell / nstep:g}, trajectory length {ell}, nstep {nstep}, "
+                               f"nstep_gauge {args.nstep_gauge}; FP32+surrogate form; "
+                               "trajectories counted in unit-length equivalents",
                    "lattice": "16x16x16x32", "traj_length": args.traj_length,
                    "nstep": nstep, "nstep_gauge": args.nstep_gauge,
-                   "step": "one outer MD step (gauge level, 2 x Nroot multi-shift solves, "
-                           "2 x Nroot fermion kicks); nstep steps = one trajectory, whose "
-                           "start and end fall inside the timed steps "
-                           f"({boundaries} in this run)",
+                   "step": "one root's multi-shift solve + fermion-force kick (with the "
+                           "gauge level before the first of each half outer step); "
+                           f"{per_traj} steps = one trajectory, whose start and end fall "
+                           f"inside the timed steps ({boundaries} in this run)",
+                   "per_hour_unit": "unit-length equivalents (MD time units) per hour",
                    "start": "thermalised U(4) 8^3x16 configuration (same couplings) tiled "
                             f"2x2x2x2 (plaquette {plaq_start[0]:.5f})",
                    "form": "FP32 operator and CG with FP64 reliable updates; FP64 forces, "
@@ -309,7 +340,7 @@ def main():
     }
     if not args.no_cpu and trajs:
         out["cpu_baseline"] = cpu_baseline(trajs[0]["cg_iters"], (2 * nstep + 1) * 2 * 15,
-                                           acc)
+                                           acc, ell)
     out["job_s"] = time.perf_counter() - t_job
     print(json.dumps(out), flush=True)
 
@@ -318,7 +349,7 @@ REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "susy_refbench_U4")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
-def cpu_baseline(iters_per_traj, force_poles, acceptance):
+def cpu_baseline(iters_per_traj, force_poles, acceptance, ell):
     """The reference itself (its U(4) MPI build, oracle/_ref/susy_refbench_U4)
     on the host's cores, timed on the full 16^3 x 32 lattice: one fermion_op,
     3 iterations of the 15-shift congrad_multi and a 2-pole fermion_force;
@@ -337,7 +368,8 @@ def cpu_baseline(iters_per_traj, force_poles, acceptance):
             if p.returncode == 0 and line:
                 r = json.loads(line[-1])
                 t_traj = iters_per_traj * r["cg_iteration_s"] + force_poles * r["force_pole_s"]
-                return {"value": 3600.0 / t_traj * acceptance, "unit": "accepted traj/hour",
+                return {"value": 3600.0 * ell / t_traj * acceptance,
+                        "unit": "accepted traj/hour (unit-length equivalents)",
                         "cores": r["ranks"], "kind": "reference",
                         "sample": (f"reference U(4) MPI build on {r['ranks']} ranks, 16^3x32: "
                                    f"fermion_op {r['fermion_op_s']:.3f} s, 15-shift CG "
@@ -351,10 +383,10 @@ def cpu_baseline(iters_per_traj, force_poles, acceptance):
             say("cpu baseline: reference run failed, using the oracle\n" + p.stdout[-2000:])
         except (OSError, subprocess.SubprocessError, ValueError, KeyError) as e:
             say(f"cpu baseline: reference run failed ({e}), using the oracle")
-    return cpu_baseline_port(iters_per_traj, force_poles, acceptance)
+    return cpu_baseline_port(iters_per_traj, force_poles, acceptance, ell)
 
 
-def cpu_baseline_port(iters_per_traj, force_poles, acceptance):
+def cpu_baseline_port(iters_per_traj, force_poles, acceptance, ell):
     """The C oracle (a restatement of the reference, single core) timed on a
     sub-volume of the same workload: per-site costs of one 15-shift CG
     iteration and one fermion-force pole on U(4) 8^3 x 16, scaled to
@@ -380,7 +412,8 @@ def cpu_baseline_port(iters_per_traj, force_poles, acceptance):
     t_pole = time.perf_counter() - t
     scale = float(np.prod(DIMS) / np.prod(sub))
     t_traj = scale * (iters_per_traj * t_cg1 + force_poles * t_pole)
-    return {"value": 3600.0 / t_traj * acceptance, "unit": "accepted traj/hour", "cores": 1,
+    return {"value": 3600.0 * ell / t_traj * acceptance,
+            "unit": "accepted traj/hour (unit-length equivalents)", "cores": 1,
             "kind": "port",
             "sample": (f"C oracle, 1 core, U(4) 8^3x16 sub-volume: {t_cg1:.2f} s per 15-shift "
                        f"CG iteration, {t_pole:.2f} s per force pole; scaled x{scale:.0f} to "

@@ -214,11 +214,17 @@ int susy_update(susy_ctx *c, int form, double *stats);
 //     74-82).  Abandons an unfinished trajectory.
 //   susy_traj_step: one outer step of update_step (update_o.c:80-113):
 //     gauge level, solves + kick, gauge level, solves + kick.  Returns the
-//     steps done (1..nstep).
+//     steps done (1..nstep); the step's device work is complete on return
+//     (as it is after susy_traj_begin and susy_traj_end).
 //   susy_traj_end: after exactly nstep steps, end action and Metropolis test
 //     (update_o.c:209-282); returns 1 accepted / 0 rejected, stats as above.
+//   susy_traj_substep: one sub-step of an outer step: the multi-shift solve
+//     and fermion kick of one root, the first of each half step also running
+//     the gauge level before it (2 Nroot sub-steps make one outer step, in
+//     the order of susy_traj_step).  Returns the sub-steps done so far.
 int susy_traj_begin(susy_ctx *c, int form);
 int susy_traj_step(susy_ctx *c);
+int susy_traj_substep(susy_ctx *c);
 int susy_traj_end(susy_ctx *c, double *stats);
 
 // Timing of the last call's kernels is available through the profiler; the

@@ -56,7 +56,7 @@ EXPORTS = [
     "susy_sg_pair_loss", "susy_sg_train", "susy_eig_dsq", "susy_save_config",
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
     "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
-    "susy_traj_begin", "susy_traj_step", "susy_traj_end", "susy_fermion_op_noalloc",
+    "susy_traj_begin", "susy_traj_step", "susy_traj_substep", "susy_traj_end", "susy_fermion_op_noalloc",
 ]
 
 _libs = {}
@@ -95,6 +95,7 @@ def load(ncol):
         "susy_actions_dev": (i, [vp, dp]), "susy_fermion_action_dev": (i, [vp, i, i, dp]),
         "susy_plaquette_dev": (i, [vp, dp, dp]), "susy_update": (i, [vp, i, dp]),
         "susy_traj_begin": (i, [vp, i]), "susy_traj_step": (i, [vp]),
+        "susy_traj_substep": (i, [vp]),
         "susy_traj_end": (i, [vp, dp]),
         "susy_fermion_op_noalloc": (i, [vp, vp, vp, i]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
@@ -301,6 +302,10 @@ class Lattice:
     def traj_step(self):
         """one outer MD step of the open trajectory; returns the steps done"""
         return self._check(self.L.susy_traj_step(self.h))
+
+    def traj_substep(self):
+        """one solve + fermion kick of one root; returns the sub-steps done"""
+        return self._check(self.L.susy_traj_substep(self.h))
 
     def traj_end(self):
         """Metropolis test of the open trajectory; returns the stats vector"""

@@ -764,7 +764,8 @@ struct Traj {
   std::vector<std::vector<double2 *>> psim;
   double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
   int iters = 0;        // CG iterations so far
-  int step = 0;         // outer steps done
+  int step = 0;         // outer steps begun
+  int sub = 0;          // sub-steps done in the current outer step
   bool open = false;    // between begin and end
   double p0[6] = {0, 0, 0, 0, 0, 0};   // start action parts
 };
@@ -861,22 +862,47 @@ static int outer_kick(Traj &T, double eps, bool solve, int *iters) {
   return 0;
 }
 
-// One outer step of update_step (update_o.c:80-113): gauge level, solves and
-// fermion kick at the midpoint, gauge level, solves and the closing kick.
-static int outer_step(Traj &T) {
+// Sub-steps of update_step's outer loop (update_o.c:80-113).  An outer step
+// is two halves -- the gauge level, then for every root the solve and its
+// fermion kick, at lambda_mid the first time and at 2 lambda (lambda on the
+// last step) the second -- and each half is Nroot sub-steps: sub-step n
+// solves and kicks root n, the first also runs the gauge level before it and
+// the last also closes the outer kick (the surrogate's removal in that form).
+// The operations and their order are those of the whole step.
+static int sub_per_half(const Traj &T) { return T.pure ? 1 : T.c->p.Nroot; }
+
+static int traj_substep(Traj &T) {
   susy_ctx *c = T.c;
   const double f_eps = c->p.traj_length / (double)c->p.nstep;
   const double g_eps = f_eps / (double)(2.0 * c->p.nstep_gauge);
-  double tr;
-  const int s = ++T.step;
-  RET(update_gauge_step(T, g_eps, &tr));
-  T.gnorm += tr;
-  if (tr > T.max_gf) T.max_gf = tr;
-  RET(outer_kick(T, f_eps * OM_LAMBDA_MID, true, &T.iters));
-  RET(update_gauge_step(T, g_eps, &tr));
-  T.gnorm += tr;
-  if (tr > T.max_gf) T.max_gf = tr;
-  RET(outer_kick(T, f_eps * (s < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA), true, &T.iters));
+  const int per = sub_per_half(T), half = T.sub / per, n = T.sub % per;
+  if (T.sub == 0) T.step++;
+  const double eps = f_eps * (half == 0 ? OM_LAMBDA_MID
+                                        : (T.step < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA));
+  if (n == 0) {
+    double tr;
+    RET(update_gauge_step(T, g_eps, &tr));
+    T.gnorm += tr;
+    if (tr > T.max_gf) T.max_gf = tr;
+    if (!T.pure) RET(sg_record_begin(c));
+  }
+  if (!T.pure) {
+    RET(traj_cg(T, n, &T.iters));
+    RET(traj_force(T, n, eps));
+    RET(sg_record_root(c));
+    if (n == per - 1) {
+      RET(sg_record_end(c));
+      if (T.sg) RET(sg_force_dev(c, -eps));
+    }
+  }
+  T.sub = (T.sub + 1) % (2 * per);
+  return 0;
+}
+
+// One outer step: gauge level, solves and fermion kick at the midpoint, gauge
+// level, solves and the closing kick.
+static int outer_step(Traj &T) {
+  for (int k = 0; k < 2 * sub_per_half(T); k++) RET(traj_substep(T));
   return 0;
 }
 
@@ -973,15 +999,17 @@ extern "C" int susy_traj_begin(susy_ctx *c, int form) {
   HIPCHK(hipMemcpyAsync(c->d_old_mom, c->d_mom, sizeof(double2) * mat5,
                         hipMemcpyDeviceToDevice, c->s));
   RET(outer_kick(T, c->p.traj_length / (double)c->p.nstep * OM_LAMBDA, false, &T.iters));
+  HIPCHK(hipStreamSynchronize(c->s));
   T.open = true;
   return 0;
 }
 
-// One outer MD step of the open trajectory; returns the steps done so far.
+// One outer MD step of the open trajectory, complete on the device when it
+// returns; returns the steps done so far.
 extern "C" int susy_traj_step(susy_ctx *c) {
   if (!c) return SUSY_E_STATE;
-  if (!c->traj || !c->traj->open || c->traj->step >= c->p.nstep) {
-    set_error("susy_traj_step: no open trajectory with steps left");
+  if (!c->traj || !c->traj->open || c->traj->step >= c->p.nstep || c->traj->sub != 0) {
+    set_error("susy_traj_step: no open trajectory at an outer-step boundary with steps left");
     return SUSY_E_STATE;
   }
   Traj &T = *c->traj;
@@ -990,7 +1018,28 @@ extern "C" int susy_traj_step(susy_ctx *c) {
     T.open = false;
     return r;
   }
+  HIPCHK(hipStreamSynchronize(c->s));
   return T.step;
+}
+
+// One sub-step of the open trajectory (the solve and fermion kick of one
+// root; see traj_substep), complete on the device when it returns; returns the
+// sub-steps done in the trajectory so far (2 Nroot per outer step).
+extern "C" int susy_traj_substep(susy_ctx *c) {
+  if (!c) return SUSY_E_STATE;
+  if (!c->traj || !c->traj->open || (c->traj->step >= c->p.nstep && c->traj->sub == 0)) {
+    set_error("susy_traj_substep: no open trajectory with sub-steps left");
+    return SUSY_E_STATE;
+  }
+  Traj &T = *c->traj;
+  int r = traj_substep(T);
+  if (r < 0) {
+    T.open = false;
+    return r;
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  const int per = 2 * sub_per_half(T);
+  return T.sub == 0 ? T.step * per : (T.step - 1) * per + T.sub;
 }
 
 // End of the trajectory (update_o.c:209-282): the end action (the last
@@ -999,7 +1048,7 @@ extern "C" int susy_traj_step(susy_ctx *c) {
 // recomputed link fields.  Returns 1 (accepted) or 0 (rejected).
 extern "C" int susy_traj_end(susy_ctx *c, double *stats) {
   if (!c) return SUSY_E_STATE;
-  if (!c->traj || !c->traj->open || c->traj->step != c->p.nstep) {
+  if (!c->traj || !c->traj->open || c->traj->step != c->p.nstep || c->traj->sub != 0) {
     set_error("susy_traj_end: the open trajectory has not made its %d steps", c->p.nstep);
     return SUSY_E_STATE;
   }

@@ -1,20 +1,27 @@
 """Thermalised start configurations for the U(4) 16^3 x 32 workload.
 
-A 16^3 x 32 lattice needs tens of unit trajectories from a cold start before
-its actions, forces and solver costs settle (the first ones have dH of order
--10^3 and solves that grow from hundreds to thousands of iterations), which is
-far more than a benchmark run can spend.  The start is therefore made once on
-an 8^3 x 16 lattice with the same couplings (lambda 2, bmass 0.4, G 0.05,
-kappa_u1 0, Nroot 2, Norder 15; trajectories of length 1, nstep 20) by this
-library's own RHMC, saved as a MILC gauge file, and tiled 2 x 2 x 2 x 2 onto
-16^3 x 32: every site then sees the local environment of an equilibrated
-configuration, and only modes longer than the small lattice start out of
-equilibrium.
+A 16^3 x 32 lattice needs many trajectories from a cold start before its
+actions, forces and solver costs settle, far more than a benchmark run can
+spend.  The start is therefore made once on an 8^3 x 16 lattice with the same
+couplings (lambda 2, bmass 0.4, G 0.05, kappa_u1 0, Nroot 2, Norder 15) by
+this library's own RHMC, saved as a MILC gauge file
+(configs/therm_u4_8x8x8x16.milc), and tiled 2 x 2 x 2 x 2 onto 16^3 x 32:
+every site then sees the local environment of an equilibrated configuration,
+and only modes longer than the small lattice start out of equilibrium.
 
-    python -m susy_amd.start_config --therm 40 --save PATH [--dims 8,8,8,16]
+The committed file was made in two stages, FP32 form throughout:
 
-thermalises from a cold start in the FP32 form (the same Markov chain as
-plain FP64 up to rounding) and prints one line per trajectory.
+    python -m susy_amd.start_config --therm 1 --seed 2024 --save A.milc
+    python -m susy_amd.start_config --load A.milc --seed 31 --therm 46 \
+        --traj-length 0.25 --nstep 20 --save therm_u4_8x8x8x16.milc
+
+The first unit trajectory from the cold links (nstep 20, dH -60.6) lands in a
+rough configuration (plaquette 5.61) from which unit trajectories at nstep 20
+have dH of +6 to +9 (a step-size error: it falls as 1/nstep^2, 6.66 / 1.64 /
+0.40 at nstep 20 / 40 / 80 on identical momenta), so the second stage runs
+at MD step 0.0125: 46 trajectories of length 0.25, all accepted, the
+plaquette falling 5.47 -> 4.69 and flat (4.67-4.70) over the last 20, where
+|dH| ~ 0.015.
 """
 import argparse
 import json
@@ -80,9 +87,15 @@ def main(argv=None):
     ap.add_argument("--save", default=None)
     ap.add_argument("--max-seconds", type=float, default=None,
                     help="start no further trajectory after this many seconds")
+    ap.add_argument("--load", default=None, help="continue from this MILC file")
+    ap.add_argument("--traj-length", type=float, default=COUPLINGS["traj_length"])
+    ap.add_argument("--nstep", type=int, default=COUPLINGS["nstep"])
     a = ap.parse_args(argv)
     dims = tuple(int(v) for v in a.dims.split(","))
-    lat = susy_amd.Lattice(4, dims=dims, iseed=a.seed, **COUPLINGS)
+    kw = dict(COUPLINGS, traj_length=a.traj_length, nstep=a.nstep)
+    lat = susy_amd.Lattice(4, dims=dims, iseed=a.seed, **kw)
+    if a.load:
+        lat.load_config(a.load)
     rows = []
     t_start = time.perf_counter()
     for k in range(a.therm):
@@ -96,7 +109,8 @@ def main(argv=None):
               f"plaq {rows[-1]['plaq']:.6f} {rows[-1]['s']:.1f} s", flush=True)
         if a.save:
             lat.save_config(a.save)
-    print(json.dumps({"dims": dims, "seed": a.seed, "trajectories": rows}), flush=True)
+    print(json.dumps({"dims": dims, "seed": a.seed, "traj_length": a.traj_length,
+                      "nstep": a.nstep, "trajectories": rows}), flush=True)
 
 
 if __name__ == "__main__":

@@ -117,3 +117,27 @@ def test_save_load_continue(oracle_chains, tmp_path):
         say("gpu continuation", k, st[:5])
         compare(st, cont[k], k)
     lat2.close()
+
+
+@pytest.mark.parametrize("form", [susy_amd.FORM_FP64, susy_amd.FORM_FP32,
+                                  susy_amd.FORM_FP64 | susy_amd.FORM_SURROGATE])
+def test_pieces_equal_the_whole_trajectory(form):
+    """a trajectory driven through susy_traj_begin / susy_traj_substep /
+    susy_traj_end is the trajectory of susy_update, bit for bit"""
+    runs = []
+    for pieces in (False, True):
+        lat = susy_amd.Lattice(2, **KW)
+        assert lat.load_config(config_path(2)) == 0
+        lat.sg_init(seed=2, scale=0.5)
+        if pieces:
+            lat.traj_begin(form)
+            n = 0
+            while n < 2 * KW["Nroot"] * KW["nstep"]:
+                n = lat.traj_substep()
+            st = lat.traj_end()
+        else:
+            st = lat.update(form)
+        runs.append((st, lat.get_links(), lat.get_mom()))
+        lat.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])

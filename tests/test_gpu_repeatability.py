@@ -69,3 +69,35 @@ def test_fp64_trajectories_repeat_bitwise():
         lat.close()
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_forces_and_link_update_repeat_bitwise():
+    """fermion_force, gauge_force + update_u (with det_force: kappa_u1 > 0) and
+    the surrogate force at U(3) 8^3 x 16, each repeated from identical inputs"""
+    kw = dict(dims=(8, 8, 8, 16), PBC=-1, iseed=5, Nroot=1, Norder=15, lam=2.0, kappa_u1=0.3,
+              bmass=0.4, G=0.05)
+    lat = susy_amd.Lattice(3, **kw)
+    U = random_links(lat.link_shape(), 11, 0.2)
+    mom = random_tf(lat.link_shape(), 12)
+    sol = random_tf((3,) + lat.tf_shape(), 13)
+    h = lat.tf_alloc(3)
+    for j in range(3):
+        lat.tf_upload(h + j, sol[j])
+    amps = lat.rhmc()[1:4]
+    lat.sg_init(seed=4, scale=1.0)
+    outs = []
+    for rep in range(3):
+        say("force repeat", rep)
+        lat.set_links(U)
+        lat.set_mom(mom)
+        nf = lat.fermion_force(0.02, h, amps)
+        m1 = lat.get_mom()
+        ng = lat.gauge_force(0.03)
+        lat.update_u(0.01)
+        U1 = lat.get_links()
+        S, F = lat.sg_eval(force=True)
+        outs.append((nf, m1, ng, U1, S, F))
+    for o in outs[1:]:
+        assert o[0] == outs[0][0] and o[2] == outs[0][2] and o[4] == outs[0][4]
+        for q in (1, 3, 5):
+            assert np.array_equal(o[q], outs[0][q]), q
