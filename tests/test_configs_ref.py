@@ -1,6 +1,9 @@
 """CPU checks of the configuration fixtures the GPU tests compare with
 (tests/golden/config0_ref.json, config1_ref.json): the oracle reproduces the
-first recorded trajectory, and the recorded solve is self-consistent."""
+first recorded trajectory, the oracle's ten trajectories of configs[0] agree
+with what the reference's own susy_hmc printed for the same run
+(tests/golden/config0_refbuild.json) to every printed digit, and the recorded
+solve is self-consistent."""
 import json
 import os
 
@@ -8,9 +11,11 @@ import numpy as np
 
 import measure_ref
 from oracle_lib import GOLDEN, Oracle
+from refcheck import check_traj, printed_ok
 
 CFG0 = json.load(open(os.path.join(GOLDEN, "config0_ref.json")))
 CFG1 = json.load(open(os.path.join(GOLDEN, "config1_ref.json")))
+CFG0_REF = json.load(open(os.path.join(GOLDEN, "config0_refbuild.json")))
 
 
 def test_config0_fixture_first_trajectory():
@@ -23,6 +28,20 @@ def test_config0_fixture_first_trajectory():
     assert list(O.plaquette()) == ref["plaq"]
     re_im, mod = measure_ref.wilson_line(measure_ref.Geometry(kw["dims"]), O.get_links(), 3)
     assert [re_im.real, re_im.imag, mod] == ref["ploop"]
+
+
+def test_config0_oracle_fixture_matches_the_reference_build():
+    """the oracle's recorded run of configs[0] (written by the oracle) against
+    the numbers the reference's susy_hmc printed for the same input"""
+    trajs = CFG0_REF["ref"]["trajectories"]
+    assert len(trajs) == len(CFG0["trajectories"]) == 10
+    for k, (tr, rec) in enumerate(zip(trajs, CFG0["trajectories"])):
+        check_traj(np.array(rec["stats"]), tr)
+        # GMES: Re, Im Polyakov loop, iterations, ss and st plaquettes (%.8g)
+        assert printed_ok(rec["ploop"][0], tr["gmes"][0], 8), k
+        assert printed_ok(rec["ploop"][1], tr["gmes"][1], 8), k
+        assert printed_ok(rec["plaq"][0], tr["gmes"][3], 8), k
+        assert printed_ok(rec["plaq"][1], tr["gmes"][4], 8), k
 
 
 def test_config1_fixture_consistency():

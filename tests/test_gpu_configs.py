@@ -1,8 +1,10 @@
 """The workload configurations of the benchmark description, as parity cases
 on the GPU:
   * U(2) 4^4 plain FP64 RHMC (lambda 1, bmass 0.5, G 0.05, Nroot 1, length 1,
-    nstep 10 / 10, cold start, seed 1234): ten trajectories against the C
-    oracle's run of the same configuration (tests/golden/config0_ref.json);
+    nstep 10 / 10, cold start, seed 1234): ten trajectories against the
+    numbers the reference's own susy_hmc printed for the same run
+    (tests/golden/config0_refbuild.json, every printed digit) and against the
+    C oracle's run (tests/golden/config0_ref.json, 1e-11);
   * U(3) 8^3 x 16 kernel level: gather tables and fermion_op against the
     oracle, the 15-shift x^(-1/4) multi-shift CG to relative residual 1e-8
     against the oracle's solve (tests/golden/config1_ref.json), and the FP32
@@ -21,11 +23,13 @@ import pytest
 import measure_ref
 import susy_amd
 from oracle_lib import GOLDEN, Oracle, random_links, random_tf, rel_err
+from refcheck import check_traj, printed_ok
 
 pytestmark = pytest.mark.gpu
 
 CFG0 = json.load(open(os.path.join(GOLDEN, "config0_ref.json")))
 CFG1 = json.load(open(os.path.join(GOLDEN, "config1_ref.json")))
+CFG0_REF = json.load(open(os.path.join(GOLDEN, "config0_refbuild.json")))
 
 
 def say(*a):
@@ -40,6 +44,13 @@ def test_config0_trajectories_match_oracle():
     for k, ref in enumerate(CFG0["trajectories"]):
         st = lat.update(susy_amd.FORM_FP64)
         say("config0 traj", k, st[:5])
+        tr = CFG0_REF["ref"]["trajectories"][k]
+        check_traj(st, tr)
+        re_, im_, _ = lat.wilson_line(3)
+        ss_, st_ = lat.plaquette()
+        for got, want_ in zip((re_, im_, ss_, st_), (tr["gmes"][0], tr["gmes"][1],
+                                                    tr["gmes"][3], tr["gmes"][4])):
+            assert printed_ok(got, want_, 8), ("gmes", k, got, want_)
         want = ref["stats"]
         assert int(st[4]) == int(want[4]), ("iterations", k)
         assert bool(st[3]) == bool(want[3]), ("accept", k)
