@@ -192,8 +192,12 @@ k_dslash_prep(const typename CT<R>::t *__restrict__ src,
 }
 
 template <typename R> struct DsSites;
-template <> struct DsSites<double> { static constexpr int n = DS_SITES64, waves = DS_WAVES64; };
-template <> struct DsSites<float> { static constexpr int n = DS_SITES32, waves = DS_WAVES32; };
+template <> struct DsSites<double> {
+  static constexpr int n = DS_SITES64, waves = DS_WAVES64, cpb = DS_CPB64;
+};
+template <> struct DsSites<float> {
+  static constexpr int n = DS_SITES32, waves = DS_WAVES32, cpb = DS_CPB32;
+};
 
 // EPI = 0: dst = result.  EPI = 1: dst = result + shift * p0, and the block's
 // partial sum of Re Tr(p0^dag dst) goes to partial[block].
@@ -208,7 +212,7 @@ template <> struct DsSites<float> { static constexpr int n = DS_SITES32, waves =
 // first stores.  Fewer sites are then in flight per XCD at the same
 // occupancy, so their neighbourhoods compete less for the XCD's L2.
 template <typename R, bool DAG, int EPI>
-__global__ void __launch_bounds__(DsSites<R>::n * DS_SPLIT)
+__global__ void __launch_bounds__(DsSites<R>::n * DS_SPLIT * DsSites<R>::cpb)
     __attribute__((amdgpu_waves_per_eu(DsSites<R>::waves)))
 k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restrict__ dst,
          const typename CT<R>::t *__restrict__ U, const typename CT<R>::t *__restrict__ Uinv,
@@ -223,11 +227,16 @@ k_dslash(const typename CT<R>::t *__restrict__ src, typename CT<R>::t *__restric
   // (the flag is one word read by every thread: the exit is block-uniform)
   if (skip != nullptr && *skip) return;
   const int L = blockIdx.x;
-  constexpr int DS_SITES = DsSites<R>::n, DS_THREADS = DS_SITES * DS_SPLIT;
-  const int lane = threadIdx.x % DS_SITES, w = threadIdx.x / DS_SITES;
+  constexpr int DS_SITES = DsSites<R>::n, CPB = DsSites<R>::cpb;
+  constexpr int DS_THREADS = DS_SITES * DS_SPLIT * CPB;
+  static_assert(CPB == 1 || DS_SPLIT == 1, "components per block need DS_SPLIT 1");
+  static_assert(NTF % CPB == 0 && DS_SITES % 64 == 0, "block shape");
+  const int lane = threadIdx.x % DS_SITES, w = CPB == 1 ? threadIdx.x / DS_SITES : 0;
   const int q = L >> 3;                       // position in this XCD's queue
-  const int comp = q % NTF;
-  const int qq = q / NTF;                     // chunk position in the queue
+  // a block holds CPB consecutive output components of one site chunk, one
+  // group of DS_SITES threads (whole wavefronts) per component
+  const int comp = CPB == 1 ? q % NTF : (q % (NTF / CPB)) * CPB + (int)threadIdx.x / DS_SITES;
+  const int qq = q / (NTF / CPB);             // chunk position in the queue
   int s;
   bool live;
   if (ord.paired) {
@@ -520,10 +529,10 @@ static int dslash_launch(susy_ctx *c, const typename CT<R>::t *src,
     HIPCHK(hipGetLastError());
   }
   DsOrder ord;
-  constexpr int threads = DsSites<R>::n * DS_SPLIT;
+  constexpr int threads = DsSites<R>::n * DS_SPLIT * DsSites<R>::cpb;
   const int nb = ds_grid(V, c->p.nx * c->p.ny * c->p.nz,
                          sizeof(R) == sizeof(float) ? c->site_order32 : c->site_order,
-                         DsSites<R>::n, &ord);
+                         DsSites<R>::n, &ord, DsSites<R>::cpb);
   if (nblocks) *nblocks = nb;
   R pbc = (R)c->p.PBC, G = (R)c->p.G;
 #define DS_ARGS src, dst, U, Uinv, pdet, aux, aux_out, c->d_nbr, c->d_t, V, c->p.nt, pbc, \

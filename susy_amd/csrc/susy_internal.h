@@ -38,6 +38,14 @@
 #ifndef DS_SPLIT
 #define DS_SPLIT 1
 #endif
+// output components per block of the operator kernel (each a whole number
+// of wavefronts over the block's site chunk); 1 = one component per block
+#ifndef DS_CPB64
+#define DS_CPB64 1
+#endif
+#ifndef DS_CPB32
+#define DS_CPB32 1
+#endif
 #define C2 1.0
 
 // Block order of the fused operator kernel.  Hardware block ids are dealt to
@@ -56,7 +64,7 @@ struct DsOrder {
 };
 // grid size (blocks) of the operator kernel, and its order parameters;
 // tslice = sites per t-slice (nx ny nz)
-inline int ds_grid(int V, int tslice, int paired, int DS_SITES, DsOrder *o) {
+inline int ds_grid(int V, int tslice, int paired, int DS_SITES, DsOrder *o, int cpb = 1) {
   DsOrder d;
   d.paired = paired;
   d.half = V / 2;
@@ -75,12 +83,12 @@ inline int ds_grid(int V, int tslice, int paired, int DS_SITES, DsOrder *o) {
       }
     }
     if (o) *o = d;
-    return 8 * d.slab * 2 * NTF;
+    return 8 * d.slab * 2 * (NTF / cpb);
   }
   const int chunks = (V + DS_SITES - 1) / DS_SITES;
   d.slab = (chunks + 7) / 8;
   if (o) *o = d;
-  return 8 * d.slab * NTF;
+  return 8 * d.slab * (NTF / cpb);
 }
 
 // ---------------------------------------------------------------------
