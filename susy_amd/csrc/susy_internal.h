@@ -233,6 +233,9 @@ struct susy_ctx {
   double2 *d_dmu = nullptr;         // 1 matrix
   double2 *d_fmunu = nullptr;       // 10 matrices
   double2 *d_force = nullptr;       // 5 matrices (force accumulator)
+  std::vector<double2 *> d_psol;    // fermion force: D psi_n per pole (on first use)
+  double2 *d_faux = nullptr;        // fermion force: 12 complex per site per pole
+  int force_npole = 0;
   // FP32 copies of the gauge-derived operands of the FP32 operator
   float2 *f_links = nullptr, *f_uinv = nullptr, *f_pdet = nullptr;
   int f_fresh = 0;                  // FP32 copies current?

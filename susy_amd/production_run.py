@@ -48,13 +48,22 @@ def main(argv=None):
     if "surrogate" in a.form:
         lat.sg_init(seed=1, scale=1.0)
         lat.sg_set_params(np.load(THETA))
+    per_traj = 2 * kw["Nroot"] * a.nstep
     rows = []
     t_start = time.perf_counter()
     for k in range(a.traj):
         if a.max_seconds is not None and time.perf_counter() - t_start > a.max_seconds:
             break
         t0 = time.perf_counter()
-        st = lat.update(FORMS[a.form])
+        # the trajectory in its pieces (the same operations as one update):
+        # start, 2 halves x Nroot roots x nstep solves with their kicks, end;
+        # a line after every solve (an FP64 solve takes 10-15 s here)
+        lat.traj_begin(FORMS[a.form])
+        for j in range(per_traj):
+            lat.traj_substep()
+            print(f"{a.form} traj {k}: solve {j + 1}/{per_traj} "
+                  f"{time.perf_counter() - t0:.1f} s", flush=True)
+        st = lat.traj_end()
         secs = time.perf_counter() - t0
         re, im, mod = lat.wilson_line(3)
         ss, stp = lat.plaquette()
