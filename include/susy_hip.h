@@ -115,6 +115,8 @@ enum {
   SUSY_FORM_PUREGAUGE = 8,       // flag: fermions off, the reference's -DPUREGAUGE build
                                  // (update_o.c:74-177, action.c:314): no pseudofermions,
                                  // solves or fermion forces; the action has no fermion part
+  SUSY_FORM_SPLIT = 16,          // flag: the large-shift poles on a middle MD level
+                                 // (susy_set_split)
 };
 
 int susy_ncol(void);
@@ -205,7 +207,9 @@ int susy_grsource_dev(susy_ctx *c, int h_src, int h_psim, int form, int *iters);
 // One RHMC trajectory (update_o.c:127-282) in the given form.
 // stats[0..22]: start S, end S, dS, accepted, CG iterations, gauge-force
 // avg/max, fermion-force avg/max for roots 0 and 1, start action parts
-// (gauge, bmass, det, fermion, mom, total), end action parts.
+// (gauge, bmass, det, fermion, mom, total), end action parts; stats[23]:
+// CG iterations of the middle level (SUSY_FORM_SPLIT; included in stats[4]).
+// stats holds 32 doubles.
 int susy_update(susy_ctx *c, int form, double *stats);
 // The same trajectory in pieces, for callers that time or interleave its
 // outer MD steps; susy_update = begin, nstep x step, end.
@@ -325,6 +329,20 @@ int susy_sg_pair_loss(susy_ctx *c, int i, double *loss);
 /* Adam over recorded pairs [first, first + count) */
 int susy_sg_train(susy_ctx *c, int first, int count, int steps, int batch, double lr, int seed,
                   double *loss_hist);
+
+/* Pole-split integrator (no counterpart in the reference, whose update_o.c
+   integrates the whole pseudofermion action on the outer level).  With
+   SUSY_FORM_SPLIT the rational action's poles j >= first_pole (the large
+   shifts, whose multi-shift solve converges in few iterations) form a middle
+   MD level: S_mid = sum_roots Phi^dag sum_{j >= first_pole} w_j (D^dag D +
+   shift_j)^(-1) Phi, integrated with nstep_mid Omelyan steps inside each
+   half outer step, each step running the gauge level (nstep_gauge steps)
+   twice.  The outer level integrates S_F - S_mid (amplitudes amp4_j for
+   j < first_pole, amp4_j - w_j above).  weights = NULL takes w_j = amp4_j
+   (the outer level keeps exactly the small-shift poles).  The Hamiltonian
+   and the Metropolis test are unchanged.  1 <= first_pole < Norder,
+   nstep_mid >= 1. */
+int susy_set_split(susy_ctx *c, int first_pole, int nstep_mid, const double *weights);
 
 #ifdef __cplusplus
 }

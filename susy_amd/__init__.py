@@ -18,6 +18,7 @@ FORM_FP64 = 0
 FORM_FP32 = 1
 FORM_SURROGATE = 4      # flag: learned surrogate force on the inner MD level
 FORM_PUREGAUGE = 8      # flag: fermions off (the reference's -DPUREGAUGE build)
+FORM_SPLIT = 16         # flag: large-shift poles on a middle MD level (set_split)
 # operator-kernel block order a new context starts with (susy_set_site_order:
 # FP64 order in the low byte, FP32 order in the next)
 DEFAULT_SITE_ORDER = 2 + (2 << 8)
@@ -57,6 +58,7 @@ EXPORTS = [
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
     "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
     "susy_traj_begin", "susy_traj_step", "susy_traj_substep", "susy_traj_end", "susy_fermion_op_noalloc",
+    "susy_set_split",
 ]
 
 _libs = {}
@@ -98,6 +100,7 @@ def load(ncol):
         "susy_traj_substep": (i, [vp]),
         "susy_traj_end": (i, [vp, dp]),
         "susy_fermion_op_noalloc": (i, [vp, vp, vp, i]),
+        "susy_set_split": (i, [vp, i, i, dp]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
         "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
         "susy_set_site_order": (i, [vp, i]),
@@ -294,6 +297,14 @@ class Lattice:
         st = np.zeros(32)
         self._check(self.L.susy_update(self.h, form, _dp(st)))
         return st
+
+    def set_split(self, first_pole, nstep_mid=1, weights=None):
+        """poles j >= first_pole on a middle MD level of nstep_mid Omelyan
+        steps per half outer step (FORM_SPLIT); weights: the middle level's
+        amplitudes for those poles (None: the rational approximation's)"""
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self._check(self.L.susy_set_split(self.h, int(first_pole), int(nstep_mid),
+                                          None if w is None else _dp(w)))
 
     def traj_begin(self, form=FORM_FP64):
         """start a trajectory (heatbath, start action, first fermion kick)"""

@@ -762,8 +762,12 @@ struct Traj {
   int form;             // precision of the CG (SUSY_FORM_FP64 / SUSY_FORM_FP32)
   bool sg = false;      // learned surrogate on the inner level
   bool pure = false;    // fermions switched off (-DPUREGAUGE)
+  bool split = false;   // large-shift poles on a middle level (susy_set_split)
   std::vector<double2 *> gsrc;
   std::vector<std::vector<double2 *>> psim;
+  std::vector<std::vector<double2 *>> psim_mid;   // middle-level solutions
+  double amp_out[16] = {0};   // outer-level amplitudes
+  int mid_iters = 0;          // middle-level CG iterations
   double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
   int iters = 0;        // CG iterations so far
   int step = 0;         // outer steps begun
@@ -803,7 +807,7 @@ static int update_gauge_step(Traj &T, double eps, double *norm) {
 
 static int traj_force(Traj &T, int root, double eps) {
   double nm;
-  RET(fermion_force_dev(T.c, eps, T.psim[root].data(), T.c->p.Norder, T.c->amp4, &nm));
+  RET(fermion_force_dev(T.c, eps, T.psim[root].data(), T.c->p.Norder, T.amp_out, &nm));
   T.fnorm[root] += nm;
   if (nm > T.max_ff[root]) T.max_ff[root] = nm;
   return 0;
@@ -846,6 +850,45 @@ static int traj_cg(Traj &T, int root, int *iters) {
   return 0;
 }
 
+// Middle-level kick of the split form: for every root the multi-shift solve
+// over the large shifts only (shift4[j], j >= split_first; it converges in
+// few iterations) and the force of those poles with the middle weights.
+static int mid_kick(Traj &T, double eps) {
+  susy_ctx *c = T.c;
+  const int j0 = c->split_first, nm = c->p.Norder - j0;
+  for (int n = 0; n < c->p.Nroot; n++) {
+    double sr, fn;
+    int it = cg_multi(c, T.gsrc[n], T.psim_mid[n].data(), nm, c->shift4 + j0, c->p.niter,
+                      c->p.error_per_site * c->p.error_per_site, &sr, T.form);
+    if (it < 0) return it;
+    T.iters += it;
+    T.mid_iters += it;
+    RET(fermion_force_dev(c, eps, T.psim_mid[n].data(), nm, c->split_w + j0, &fn));
+  }
+  return 0;
+}
+
+// Middle level of the split form: split_nmid Omelyan steps advancing the
+// links by len, the middle kicks at the step points and the whole gauge
+// level (advancing len / (2 split_nmid)) between them.
+static int mid_level(Traj &T, double len, double *norm) {
+  susy_ctx *c = T.c;
+  const int n = c->split_nmid;
+  const double m = len / n, g = m / (2.0 * c->p.nstep_gauge);
+  double tr, acc = 0;
+  RET(mid_kick(T, m * OM_LAMBDA));
+  for (int i = 1; i <= n; i++) {
+    RET(update_gauge_step(T, g, &tr));
+    acc += tr;
+    RET(mid_kick(T, m * OM_LAMBDA_MID));
+    RET(update_gauge_step(T, g, &tr));
+    acc += tr;
+    RET(mid_kick(T, m * (i < n ? OM_TWO_LAMBDA : OM_LAMBDA)));
+  }
+  *norm = acc / (2 * n);
+  return 0;
+}
+
 // Outer (fermion) kick of size eps at the current links: the solves (unless
 // the caller's psim already belong to these links), every root's force, and
 // in the surrogate mode the removal of the learned force that the inner
@@ -883,7 +926,10 @@ static int traj_substep(Traj &T) {
                                         : (T.step < c->p.nstep ? OM_TWO_LAMBDA : OM_LAMBDA));
   if (n == 0) {
     double tr;
-    RET(update_gauge_step(T, g_eps, &tr));
+    if (T.split)
+      RET(mid_level(T, 0.5 * f_eps, &tr));
+    else
+      RET(update_gauge_step(T, g_eps, &tr));
     T.gnorm += tr;
     if (tr > T.max_gf) T.max_gf = tr;
     if (!T.pure) RET(sg_record_begin(c));
@@ -941,14 +987,20 @@ __global__ void k_neg(double2 *__restrict__ y, const double2 *__restrict__ x, si
 // is abandoned (links and momenta stay as they are).
 extern "C" int susy_traj_begin(susy_ctx *c, int form) {
   if (!c) return SUSY_E_STATE;
-  if ((form & ~(SUSY_FORM_FP32 | SUSY_FORM_SURROGATE | SUSY_FORM_PUREGAUGE)) != 0) {
+  if ((form & ~(SUSY_FORM_FP32 | SUSY_FORM_SURROGATE | SUSY_FORM_PUREGAUGE | SUSY_FORM_SPLIT)) !=
+      0) {
     set_error("susy_traj_begin: unknown form bits 0x%x", form);
     return SUSY_E_ARG;
   }
   const bool sg = (form & SUSY_FORM_SURROGATE) != 0, pure = (form & SUSY_FORM_PUREGAUGE) != 0;
-  if (sg && pure) {
-    set_error("susy_traj_begin: the surrogate form needs the fermions");
+  const bool split = (form & SUSY_FORM_SPLIT) != 0;
+  if ((sg || split) && pure) {
+    set_error("susy_traj_begin: the surrogate and split forms need the fermions");
     return SUSY_E_ARG;
+  }
+  if (split && c->split_first < 1) {
+    set_error("susy_traj_begin: split form without susy_set_split");
+    return SUSY_E_STATE;
   }
   if (sg && !c->sg) {
     set_error("susy_traj_begin: surrogate form without susy_sg_init");
@@ -965,11 +1017,15 @@ extern "C" int susy_traj_begin(susy_ctx *c, int form) {
   T.form = form & SUSY_FORM_FP32;
   T.sg = sg;
   T.pure = pure;
+  T.split = split;
   const int Nr = c->p.Nroot, No = c->p.Norder;
+  const int nmid = split ? No - c->split_first : 0;
+  for (int j = 0; j < No; j++)
+    T.amp_out[j] = split && j >= c->split_first ? c->amp4[j] - c->split_w[j] : c->amp4[j];
   if (!pure) {
     // per-trajectory fields are kept in the context across calls
     std::vector<double2 *> &pool = c->traj_pool;
-    size_t need = (size_t)Nr * (1 + No);
+    size_t need = (size_t)Nr * (1 + No + nmid);
     while (pool.size() < need) {
       double2 *p;
       RET(dalloc(&p, tf_elems(c)));
@@ -980,6 +1036,11 @@ extern "C" int susy_traj_begin(susy_ctx *c, int form) {
     for (int n = 0; n < Nr; n++) {
       T.gsrc[n] = pool[n * (1 + No)];
       T.psim[n].assign(pool.begin() + n * (1 + No) + 1, pool.begin() + (n + 1) * (1 + No));
+    }
+    T.psim_mid.resize(Nr);
+    for (int n = 0; n < Nr; n++) {
+      auto first = pool.begin() + Nr * (1 + No) + n * nmid;
+      T.psim_mid[n].assign(first, first + nmid);
     }
   }
   RET(ranmom(c));
@@ -1086,8 +1147,22 @@ extern "C" int susy_traj_end(susy_ctx *c, double *stats) {
       stats[8 + 2 * n] = n < Nr && !T.pure ? T.max_ff[n] : 0;
     }
     for (int k = 0; k < 6; k++) { stats[11 + k] = p0[k]; stats[17 + k] = p1[k]; }
+    stats[23] = T.mid_iters;
   }
   return accept;
+}
+
+extern "C" int susy_set_split(susy_ctx *c, int first_pole, int nstep_mid, const double *weights) {
+  if (!c) return SUSY_E_STATE;
+  if (first_pole < 1 || first_pole >= c->p.Norder || nstep_mid < 1) {
+    set_error("susy_set_split: need 1 <= first_pole < %d and nstep_mid >= 1", c->p.Norder);
+    return SUSY_E_ARG;
+  }
+  c->split_first = first_pole;
+  c->split_nmid = nstep_mid;
+  for (int j = 0; j < c->p.Norder; j++)
+    c->split_w[j] = j < first_pole ? 0.0 : weights ? weights[j - first_pole] : c->amp4[j];
+  return 0;
 }
 
 extern "C" int susy_update(susy_ctx *c, int form, double *stats) {

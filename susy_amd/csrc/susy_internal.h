@@ -236,6 +236,10 @@ struct susy_ctx {
   std::vector<double2 *> d_psol;    // fermion force: D psi_n per pole (on first use)
   double2 *d_faux = nullptr;        // fermion force: 12 complex per site per pole
   int force_npole = 0;
+  // pole-split integrator (susy_set_split): first middle-level pole, middle
+  // steps per half outer step, middle-level weights (entries >= split_first)
+  int split_first = 0, split_nmid = 1;
+  double split_w[16] = {0};
   // FP32 copies of the gauge-derived operands of the FP32 operator
   float2 *f_links = nullptr, *f_uinv = nullptr, *f_pdet = nullptr;
   int f_fresh = 0;                  // FP32 copies current?
