@@ -230,6 +230,9 @@ int susy_traj_begin(susy_ctx *c, int form);
 int susy_traj_step(susy_ctx *c);
 int susy_traj_substep(susy_ctx *c);
 int susy_traj_end(susy_ctx *c, double *stats);
+// CG iterations of the current (or last) trajectory so far: all, and those of
+// the middle level of SUSY_FORM_SPLIT (either pointer may be NULL).
+int susy_traj_iters(susy_ctx *c, int *iters, int *mid_iters);
 
 // Timing of the last call's kernels is available through the profiler; the
 // benchmark measures operator launches with these helpers (HIP events on the
@@ -341,7 +344,10 @@ int susy_sg_train(susy_ctx *c, int first, int count, int steps, int batch, doubl
    j < first_pole, amp4_j - w_j above).  weights = NULL takes w_j = amp4_j
    (the outer level keeps exactly the small-shift poles).  The Hamiltonian
    and the Metropolis test are unchanged.  1 <= first_pole < Norder,
-   nstep_mid >= 1. */
+   nstep_mid >= 1.  Together with SUSY_FORM_SURROGATE the learned force still
+   enters every gauge-level force and leaves every outer kick, so it stands
+   in for the outer level's force only: train it on the pairs that
+   susy_sg_record records in the split form (the outer forces). */
 int susy_set_split(susy_ctx *c, int first_pole, int nstep_mid, const double *weights);
 
 #ifdef __cplusplus

@@ -58,7 +58,7 @@ EXPORTS = [
     "susy_link_trace_dev", "susy_wilson_line_dev", "susy_scalar_eig_dev",
     "susy_plaq_moments_dev", "susy_pfaffian_dev", "susy_fermion_matrix_dev",
     "susy_traj_begin", "susy_traj_step", "susy_traj_substep", "susy_traj_end", "susy_fermion_op_noalloc",
-    "susy_set_split",
+    "susy_set_split", "susy_traj_iters",
 ]
 
 _libs = {}
@@ -101,6 +101,7 @@ def load(ncol):
         "susy_traj_end": (i, [vp, dp]),
         "susy_fermion_op_noalloc": (i, [vp, vp, vp, i]),
         "susy_set_split": (i, [vp, i, i, dp]),
+        "susy_traj_iters": (i, [vp, ip, ip]),
         "susy_time_fermion_op": (i, [vp, i, i, i, i, dp]),
         "susy_prof_enable": (i, [vp, i]), "susy_reseed": (i, [vp, i]), "susy_set_verbose": (i, [i]),
         "susy_set_site_order": (i, [vp, i]),
@@ -317,6 +318,12 @@ class Lattice:
     def traj_substep(self):
         """one solve + fermion kick of one root; returns the sub-steps done"""
         return self._check(self.L.susy_traj_substep(self.h))
+
+    def traj_iters(self):
+        """CG iterations of the current trajectory so far: (all, middle level)"""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.L.susy_traj_iters(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def traj_end(self):
         """Metropolis test of the open trajectory; returns the stats vector"""

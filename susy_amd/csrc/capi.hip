@@ -1152,6 +1152,13 @@ extern "C" int susy_traj_end(susy_ctx *c, double *stats) {
   return accept;
 }
 
+extern "C" int susy_traj_iters(susy_ctx *c, int *iters, int *mid_iters) {
+  if (!c || !c->traj) return SUSY_E_STATE;
+  if (iters) *iters = c->traj->iters;
+  if (mid_iters) *mid_iters = c->traj->mid_iters;
+  return 0;
+}
+
 extern "C" int susy_set_split(susy_ctx *c, int first_pole, int nstep_mid, const double *weights) {
   if (!c) return SUSY_E_STATE;
   if (first_pole < 1 || first_pole >= c->p.Norder || nstep_mid < 1) {

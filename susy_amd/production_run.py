@@ -8,7 +8,10 @@ action per site.  Every form started with the same --seed draws the same
 momenta and pseudofermions for its first trajectory.
 
     python -m susy_amd.production_run --form fp32_surrogate --traj 5 [--seed 1234]
-        [--max-seconds S]
+        [--max-seconds S] [--nstep N] [--split-first J [--split-nmid M]]
+
+--split-first J puts the poles J..14 of the rational action on a middle MD
+level of M Omelyan steps per half outer step (the split form).
 
 Prints one line per trajectory and a final JSON line (per trajectory: dH,
 accept, CG iterations, seconds, observables; and the acceptance, mean
@@ -41,13 +44,21 @@ def main(argv=None):
     ap.add_argument("--nstep", type=int, default=5)
     ap.add_argument("--max-seconds", type=float, default=None,
                     help="start no further trajectory after this many seconds")
+    ap.add_argument("--split-first", type=int, default=0,
+                    help="first pole of the middle level (0: no split)")
+    ap.add_argument("--split-nmid", type=int, default=1)
+    ap.add_argument("--theta", default=THETA, help="surrogate parameters (.npy)")
     a = ap.parse_args(argv)
     kw = dict(start_config.COUPLINGS, niter=20000, traj_length=a.traj_length, nstep=a.nstep)
     lat = susy_amd.Lattice(4, dims=DIMS, iseed=a.seed, **kw)
     lat.set_links(start_config.tiled_start(susy_amd, DIMS))
+    form = FORMS[a.form]
     if "surrogate" in a.form:
         lat.sg_init(seed=1, scale=1.0)
-        lat.sg_set_params(np.load(THETA))
+        lat.sg_set_params(np.load(a.theta))
+    if a.split_first:
+        lat.set_split(a.split_first, a.split_nmid)
+        form |= susy_amd.FORM_SPLIT
     per_traj = 2 * kw["Nroot"] * a.nstep
     rows = []
     t_start = time.perf_counter()
@@ -58,7 +69,7 @@ def main(argv=None):
         # the trajectory in its pieces (the same operations as one update):
         # start, 2 halves x Nroot roots x nstep solves with their kicks, end;
         # a line after every solve (an FP64 solve takes 10-15 s here)
-        lat.traj_begin(FORMS[a.form])
+        lat.traj_begin(form)
         for j in range(per_traj):
             lat.traj_substep()
             print(f"{a.form} traj {k}: solve {j + 1}/{per_traj} "
@@ -68,6 +79,7 @@ def main(argv=None):
         re, im, mod = lat.wilson_line(3)
         ss, stp = lat.plaquette()
         rows.append({"dH": float(st[2]), "accept": int(st[3]), "cg_iters": int(st[4]),
+                     "mid_iters": int(st[23]),
                      "s": secs, "ploop": [re, im, mod], "plaq": [ss, stp],
                      "sb_per_site": float(lat.actions()[0] / lat.V)})
         print(f"{a.form} traj {k}: dH {st[2]:.5g} accept {int(st[3])} iters {int(st[4])} "
@@ -77,7 +89,10 @@ def main(argv=None):
     print(json.dumps({
         "workload": "U(4) 16^3x32, lambda 2, bmass 0.4, G 0.05, kappa_u1 0, Nroot 2, "
                     f"traj length {a.traj_length}, nstep {a.nstep}, nstep_gauge 10",
-        "form": a.form, "seed": a.seed, "trajectories": rows,
+        "form": a.form, "seed": a.seed,
+        "split": {"first_pole": a.split_first, "nstep_mid": a.split_nmid}
+        if a.split_first else None,
+        "trajectories": rows,
         "acceptance": float(np.mean([r["accept"] for r in rows])) if rows else None,
         "metropolis_probability": float(np.mean(np.minimum(1.0, np.exp(-dH))))
         if rows else None,

@@ -17,6 +17,11 @@ pairs affordable:
      over all pairs but the last `--held`, which are held out;
   3. the parameters (float32, the order of susy_sg_get_params) are saved.
 
+With `--split-first J` the trajectories run in the pole-split form (poles
+J..14 on a middle MD level of `--split-nmid` steps): the recorded force is
+then the outer level's (the small-shift poles), which is what the surrogate
+stands in for on the inner level when it is used together with the split.
+
     python -m susy_amd.surrogate_train --traj 30 --pairs 500 --adam 5000 \\
         --save susy_amd/configs/surrogate_u4_theta.npy
 """
@@ -46,19 +51,25 @@ def main(argv=None):
     ap.add_argument("--save", default=None)
     ap.add_argument("--max-seconds", type=float, default=None,
                     help="start no further trajectory after this many seconds")
+    ap.add_argument("--split-first", type=int, default=0)
+    ap.add_argument("--split-nmid", type=int, default=1)
     a = ap.parse_args(argv)
 
     kw = dict(start_config.COUPLINGS, traj_length=a.traj_length, nstep=a.nstep)
     lat = susy_amd.Lattice(4, dims=start_config.SMALL_DIMS, iseed=a.seed, **kw)
     lat.load_config(start_config.START_FILE)
     lat.sg_init(seed=1, scale=1.0)
+    form = susy_amd.FORM_FP32
+    if a.split_first:
+        lat.set_split(a.split_first, a.split_nmid)
+        form |= susy_amd.FORM_SPLIT
     lat.sg_record(a.pairs)
     t0 = time.perf_counter()
     acc, done = 0, 0
     for k in range(a.traj):
         if a.max_seconds is not None and time.perf_counter() - t0 > a.max_seconds:
             break
-        st = lat.update(susy_amd.FORM_FP32)
+        st = lat.update(form)
         acc += int(st[3])
         done += 1
         say(f"generate traj {k}: dH {st[2]:.5f} accept {int(st[3])} iters {int(st[4])} "
