@@ -34,34 +34,45 @@ steps of the first trajectory into the next, and the end of one trajectory
 and the start of the next fall inside the timed region.
 
 Forms:
-  * FP32 + surrogate (headline `value`): FP32 operator and multi-shift CG with
-    FP64 reliable updates, FP64 forces, actions and dH, and the learned
-    surrogate force on the inner (gauge) MD level; `--warmup` untimed steps,
-    then `--steps` timed steps, trajectories completed and Metropolis-tested
-    as they end.
-  * FP32 (no surrogate) and plain FP64: the start of one trajectory and
-    `--fp32-steps` / `--fp64-steps` timed steps from the same links and
+  * FP32 + surrogate + pole split (headline `value`): FP32 operator and
+    multi-shift CG with FP64 reliable updates, FP64 forces, actions and dH,
+    the learned surrogate force on the inner (gauge) MD level, and the
+    rational action's large-shift poles (9..14, shifts >= 0.06) on a middle
+    MD level (one Omelyan step per half outer step, a multi-shift solve over
+    those six shifts only: 35-175 CG iterations) so that the outer level,
+    which keeps the small-shift poles, takes 3 steps per trajectory where the
+    plain forms take 5; `--warmup` untimed steps, then `--steps` timed steps,
+    trajectories completed and Metropolis-tested as they end.
+    `--split-first 0` runs the FP32 + surrogate form without the split at
+    the plain forms' nstep.
+  * FP32 (no surrogate) and plain FP64, nstep 5: the start of one trajectory
+    and `--fp32-steps` / `--fp64-steps` timed steps from the same links and
     seed; their trajectory time is start + 2 Nroot nstep x the mean step
     time.
-The surrogate's parameters come from `python -m susy_amd.surrogate_train`
-(committed as susy_amd/configs/surrogate_u4_theta.npy): the library's own
-forward and backward kernels trained it (Adam) on (links, fermion force)
-pairs of the plain RHMC on the 8^3 x 16 lattice of the same couplings -- the
-network is local, so its parameters carry over to 16^3 x 32.  The run
-reports its force error on the pairs the FP32 leg records at this size
-(pairs it was not trained on), and can fine-tune on them (`--sg-steps`).
+The surrogate's parameters come from `python -m susy_amd.surrogate_train`:
+the library's own forward and backward kernels trained them (Adam) on
+(links, outer-level fermion force) pairs of the same form's RHMC on the
+8^3 x 16 lattice of the same couplings (susy_amd/configs/
+surrogate_u4_split_theta.npy with the split, whose outer level keeps only the
+small-shift poles; surrogate_u4_theta.npy without) -- the network is local,
+so its parameters carry over to 16^3 x 32.  The run reports their force
+error on the pairs its own warm-up steps record at this size (pairs they
+were not trained on).
 
-value = accepted trajectories per hour of the FP32+surrogate form, in
-unit-length equivalents = 3600 x acceptance x traj_length x (timed steps /
-steps per trajectory) / timed seconds.  The acceptance is
-the mean Metropolis probability min(1, exp(-dH)) of the trajectories that
-ended in the run (its expectation is the acceptance rate, with far less
-variance than the 0/1 outcomes of the one or two trajectories a run of this
-length ends; the outcomes and dH are reported beside it).  The plain FP64
-and FP32 forms integrate the same Hamiltonian with the same integrator (their
-dH on identical inputs agree to ~1e-4), so they share an acceptance rate and
-are compared by trajectories per hour; the ratios to the plain FP64 form are
-reported.
+value = accepted trajectories per hour of the headline form, in unit-length
+equivalents = 3600 x acceptance x traj_length x (timed steps / steps per
+trajectory) / timed seconds.  The acceptance is the mean Metropolis
+probability min(1, exp(-dH)) of the trajectories that ended in the run (its
+expectation is the acceptance rate, with far less variance than the 0/1
+outcomes of the one or two trajectories a run of this length ends; the
+outcomes and dH are reported beside it).  The plain FP64 and FP32 forms
+integrate the same Hamiltonian with the same integrator (their dH on
+identical inputs agree to ~1e-4), so they share an acceptance rate; the
+split's 3 outer steps were chosen for an energy error no larger than the
+plain forms' at 5 (paired trajectories), so all forms are compared by
+trajectories per hour, and the ratios to the plain FP64 form are reported.
+The CPU baseline prices a trajectory of the reference (plain algorithm) at
+the FP64 leg's CG iterations.
 
 `--kernels` runs only a short operator workload (5 field copies for the
 memory-counter calibration, then 10 launches of the operator in each
@@ -89,6 +100,12 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"k_dslash_fp32_{WORKLOAD}_coun
 SEED = 1234
 # surrogate parameters trained by susy_amd/surrogate_train.py
 SG_THETA = os.path.join(ROOT, "susy_amd", "configs", "surrogate_u4_theta.npy")
+# ... and on the split form's outer forces (the headline form with the split)
+SG_THETA_SPLIT = os.path.join(ROOT, "susy_amd", "configs", "surrogate_u4_split_theta.npy")
+# headline form's pole split: poles SPLIT_FIRST..14 (shifts >= 0.06) on a
+# middle level of SPLIT_NMID Omelyan steps per half outer step, SPLIT_NSTEP
+# outer steps per trajectory of the same length
+SPLIT_FIRST, SPLIT_NMID, SPLIT_NSTEP = 9, 1, 3
 
 
 def workload(traj_length, nstep, nstep_gauge):
@@ -135,6 +152,7 @@ def partial_form(lat, susy_amd, form, name, start, nsteps, prof_every):
     t0 = time.perf_counter()
     lat.traj_begin(form)
     t_begin = time.perf_counter() - t0
+    it_begin = lat.traj_iters()[0]
     say(f"{name}: start of trajectory {t_begin:.2f} s")
     steps = []
     for k in range(nsteps):
@@ -142,9 +160,11 @@ def partial_form(lat, susy_amd, form, name, start, nsteps, prof_every):
         lat.traj_substep()
         steps.append(time.perf_counter() - t)
         say(f"{name}: step {k + 1} {steps[-1]:.2f} s")
+    it_steps = lat.traj_iters()[0] - it_begin
     prof = lat.prof_read(form & 1)
     lat.prof_enable(0)
-    return {"s_begin": t_begin, "s_steps": steps, "dslash_ms": prof[0]}
+    return {"s_begin": t_begin, "s_steps": steps, "dslash_ms": prof[0],
+            "cg_iters_begin": it_begin, "cg_iters_per_step": it_steps / max(nsteps, 1)}
 
 
 def main():
@@ -162,8 +182,12 @@ def main():
                     help="time every n-th operator kernel launch with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="operator launches only")
-    ap.add_argument("--sg-steps", type=int, default=0,
-                    help="Adam steps fine-tuning the surrogate on this run's force pairs")
+    ap.add_argument("--split-first", type=int, default=SPLIT_FIRST,
+                    help="headline form: first pole of the middle MD level (0: no split)")
+    ap.add_argument("--split-nmid", type=int, default=SPLIT_NMID,
+                    help="headline form: middle Omelyan steps per half outer step")
+    ap.add_argument("--split-nstep", type=int, default=SPLIT_NSTEP,
+                    help="headline form: outer steps per trajectory with the split")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -183,7 +207,10 @@ def main():
         return
     susy_amd.load(NCOL).susy_set_verbose(1)   # one progress line per solve (stderr)
     V, nstep = lat.V, args.nstep
+    split = args.split_first > 0
+    h_nstep = args.split_nstep if split else nstep   # the headline form's outer steps
     per_traj = 2 * 2 * nstep          # steps per trajectory: 2 halves x Nroot roots x nstep
+    h_per_traj = 2 * 2 * h_nstep
     ell = args.traj_length
 
     # ---- start configuration: thermalised 8^3 x 16 links tiled onto 16^3 x 32
@@ -202,68 +229,69 @@ def main():
            for nm, f in (("fp64", 0), ("fp32", 1))}
     say(f"isolated D: FP64 {iso['fp64']:.3f} ms, FP32 {iso['fp32']:.3f} ms")
 
-    # ---- FP32 and FP64 forms: start + a few steps each; the FP32 leg records
-    # its (links, fermion force) pairs, on which the surrogate is evaluated
-    npair_cap = 1 + args.fp32_steps
-    lat.sg_init(seed=1, scale=1.0)
-    theta = np.load(SG_THETA)
-    lat.sg_set_params(theta)
-    lat.sg_record(npair_cap)
+    # ---- FP32 and FP64 forms: start + a few steps each
     f32 = partial_form(lat, susy_amd, susy_amd.FORM_FP32, "fp32", start, args.fp32_steps,
                        args.prof_every)
-    lat.sg_record(0)
     f64 = partial_form(lat, susy_amd, susy_amd.FORM_FP64, "fp64", start, args.fp64_steps,
                        args.prof_every)
 
-    # ---- the surrogate (trained offline on the 8^3 x 16 chain of the same
-    # couplings): force error on this lattice's pairs, optional fine-tuning
-    npairs = lat.sg_pairs()
-    err_pairs = [lat.sg_pair_loss(i) for i in range(npairs)]
-    t = time.perf_counter()
-    if args.sg_steps > 0:
-        lat.sg_train(0, npairs, steps=args.sg_steps, batch=2, lr=1e-3, seed=3)
-    t_train = time.perf_counter() - t
-    say(f"surrogate: relative force error^2 on this run's {npairs} pairs "
-        f"{float(np.mean(err_pairs)):.4f}; {args.sg_steps} fine-tuning steps")
-
-    # ---- headline: FP32 + surrogate, warmup steps then timed steps
+    # ---- headline: FP32 + surrogate (+ pole split), warmup steps then timed
+    # steps; with the split, a context of its own outer step count.  The
+    # surrogate's parameters were trained offline on the 8^3 x 16 chain of the
+    # same couplings and form (the outer level's forces); the warm-up steps
+    # record this lattice's (links, outer force) pairs, on which its force
+    # error is reported.
     form = susy_amd.FORM_FP32 | susy_amd.FORM_SURROGATE
+    sg_theta = SG_THETA_SPLIT if split else SG_THETA
+    if split:
+        lat.close()
+        lat = susy_amd.Lattice(NCOL, **workload(args.traj_length, h_nstep, args.nstep_gauge))
+        lat.set_split(args.split_first, args.split_nmid)
+        form |= susy_amd.FORM_SPLIT
+    lat.sg_init(seed=1, scale=1.0)
+    lat.sg_set_params(np.load(sg_theta))
     lat.set_links(start)
     lat.reseed(SEED)
     trajs = []
-    done = [0]   # outer steps of the open trajectory
+    done = [0]   # steps of the open trajectory
+    h_name = "fp32+surrogate+split" if split else "fp32+surrogate"
 
     def one_step():
         if done[0] == 0:
             lat.traj_begin(form)
         lat.traj_substep()
         done[0] += 1
-        if done[0] == per_traj:
+        if done[0] == h_per_traj:
             st = lat.traj_end()
             plaq = lat.plaquette()
             trajs.append({"dH": float(st[2]), "accept": int(st[3]), "cg_iters": int(st[4]),
-                          "plaquette": list(plaq)})
-            say(f"fp32+surrogate: trajectory {len(trajs)} dH {st[2]:.5g} accept {int(st[3])} "
-                f"iters {int(st[4])} plaq {plaq[0]:.6f}")
+                          "middle_cg_iters": int(st[23]), "plaquette": list(plaq)})
+            say(f"{h_name}: trajectory {len(trajs)} dH {st[2]:.5g} accept {int(st[3])} "
+                f"iters {int(st[4])} (middle {int(st[23])}) plaq {plaq[0]:.6f}")
             done[0] = 0
 
+    lat.sg_record(max(args.warmup, 1))
     for k in range(args.warmup):
         one_step()
-        say(f"fp32+surrogate: warm-up step {k + 1}")
+        say(f"{h_name}: warm-up step {k + 1}")
+    lat.sg_record(0)
     lat.prof_enable(args.prof_every)
     ntraj0 = len(trajs)
     t0 = sync_time()
     for k in range(args.steps):
         one_step()
-        say(f"fp32+surrogate: timed step {k + 1} t {sync_time() - t0:.1f} s")
+        say(f"{h_name}: timed step {k + 1} t {sync_time() - t0:.1f} s")
     t_timed = sync_time() - t0
     prof = lat.prof_read(1)
     lat.prof_enable(0)
+    err_pairs = [lat.sg_pair_loss(i) for i in range(lat.sg_pairs())]
+    say(f"surrogate: relative force error^2 on the {len(err_pairs)} pairs of the warm-up "
+        f"steps {float(np.mean(err_pairs)) if err_pairs else float('nan'):.4f}")
     ended = trajs[ntraj0:]
     boundaries = len(ended)
 
     # ---- figures
-    traj_per_hour_sg = 3600.0 * ell * (args.steps / per_traj) / t_timed
+    traj_per_hour_sg = 3600.0 * ell * (args.steps / h_per_traj) / t_timed
     # acceptance: mean Metropolis probability of the ended trajectories
     acc = float(np.mean([min(1.0, np.exp(min(-tr["dH"], 700.0))) for tr in trajs])) \
         if trajs else None
@@ -278,8 +306,9 @@ def main():
     bytes32 = dslash_bytes(V, 8)
     avg_ms = prof[0]
     achieved = bytes32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    h_key = "fp32_surrogate_split" if split else "fp32_surrogate"
     forms = {
-        "fp32_surrogate": {
+        h_key: {
             "traj_per_hour": traj_per_hour_sg, "accepted_traj_per_hour": value,
             "acceptance": acc,
             "acceptance_estimator": "mean of min(1, exp(-dH)) over the ended trajectories",
@@ -288,19 +317,22 @@ def main():
             "trajectories": trajs,
             "exp_minus_dH": finite_or_none(np.mean([np.exp(min(-tr["dH"], 700.0))
                                                     for tr in trajs])) if trajs else None,
-            "s_per_traj": t_timed * per_traj / args.steps,
+            "s_per_traj": t_timed * h_per_traj / args.steps,
+            "nstep": h_nstep,
+            "split": {"first_pole": args.split_first, "nstep_mid": args.split_nmid} if split else None,
             "dslash_ms": avg_ms,
-            "surrogate": {"parameters": os.path.relpath(SG_THETA, ROOT),
+            "surrogate": {"parameters": os.path.relpath(sg_theta, ROOT),
                           "trained": "offline, susy_amd/surrogate_train.py on the 8^3x16 "
-                                     "chain of the same couplings",
-                          "force_err2_on_run_pairs": [float(x) for x in err_pairs],
-                          "fine_tuning_steps": args.sg_steps, "fine_tuning_s": t_train}},
+                                     "chain of the same couplings and form",
+                          "force_err2_on_warmup_pairs": [float(x) for x in err_pairs]}},
         "fp32": {"traj_per_hour": tph32, "s_per_traj": s32, "s_begin": f32["s_begin"],
-                 "s_steps": f32["s_steps"], "dslash_ms": f32["dslash_ms"]},
+                 "s_steps": f32["s_steps"], "dslash_ms": f32["dslash_ms"], "nstep": nstep},
         "fp64": {"traj_per_hour": tph64, "s_per_traj": s64, "s_begin": f64["s_begin"],
-                 "s_steps": f64["s_steps"], "dslash_ms": f64["dslash_ms"]},
+                 "s_steps": f64["s_steps"], "dslash_ms": f64["dslash_ms"], "nstep": nstep,
+                 "cg_iters_begin": f64["cg_iters_begin"],
+                 "cg_iters_per_step": f64["cg_iters_per_step"]},
     }
-    forms["traj_per_hour_ratio_fp32_surrogate_vs_fp64"] = traj_per_hour_sg / tph64
+    forms[f"traj_per_hour_ratio_{h_key}_vs_fp64"] = traj_per_hour_sg / tph64
     forms["traj_per_hour_ratio_fp32_vs_fp64"] = tph32 / tph64
     out = {
         "metric": "accepted traj/hour, U(4) 16^3x32: FP32+surrogate vs plain FP64; Dslash GB/s",
@@ -313,20 +345,27 @@ def main():
                                 "tiled to 16^3x32; generators seeded 1234)",
         "config": {"workload": "4d_Q16 RHMC U(4) 16^3x32, lambda 2.0, bmass 0.4, G 0.05, "
                                "kappa_u1 0, Nroot 2, Norder 15; MD step "
-                               f"{ell / nstep:g}, trajectory length {ell}, nstep {nstep}, "
-                               f"nstep_gauge {args.nstep_gauge}; FP32+surrogate form; "
-                               "trajectories counted in unit-length equivalents",
+                               f"{ell / h_nstep:g}, trajectory length {ell}, nstep {h_nstep}, "
+                               f"nstep_gauge {args.nstep_gauge}; {h_name} form"
+                               + (f" (poles {args.split_first}-14 on a middle level of "
+                                  f"{args.split_nmid} step(s); the plain FP32 and FP64 forms "
+                                  f"at nstep {nstep})" if split else "")
+                               + "; trajectories counted in unit-length equivalents",
                    "lattice": "16x16x16x32", "traj_length": args.traj_length,
-                   "nstep": nstep, "nstep_gauge": args.nstep_gauge,
+                   "nstep": h_nstep, "nstep_gauge": args.nstep_gauge,
                    "step": "one root's multi-shift solve + fermion-force kick (with the "
-                           "gauge level before the first of each half outer step); "
-                           f"{per_traj} steps = one trajectory, whose start and end fall "
+                           + ("middle and gauge levels" if split else "gauge level")
+                           + " before the first of each half outer step); "
+                           f"{h_per_traj} steps = one trajectory, whose start and end fall "
                            f"inside the timed steps ({boundaries} in this run)",
                    "per_hour_unit": "unit-length equivalents (MD time units) per hour",
                    "start": "thermalised U(4) 8^3x16 configuration (same couplings) tiled "
                             f"2x2x2x2 (plaquette {plaq_start[0]:.5f})",
                    "form": "FP32 operator and CG with FP64 reliable updates; FP64 forces, "
-                           "actions and dH; learned surrogate force on the inner MD level"},
+                           "actions and dH; learned surrogate force on the inner MD level"
+                           + ("; rational-action poles with shifts >= "
+                              f"{lat.rhmc()[16 + args.split_first]:.3g} on a middle MD level"
+                              if split else "")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(),
@@ -339,8 +378,11 @@ def main():
                             "fp32_ms": iso["fp32"], "fp32_GBs": bytes32 / iso["fp32"] / 1e6},
     }
     if not args.no_cpu and trajs:
-        out["cpu_baseline"] = cpu_baseline(trajs[0]["cg_iters"], (2 * nstep + 1) * 2 * 15,
-                                           acc, ell)
+        # the reference runs the plain algorithm: a trajectory is priced at the
+        # FP64 leg's CG iterations (start + per_traj steps) and the plain
+        # integrator's force poles
+        it64 = int(round(f64["cg_iters_begin"] + per_traj * f64["cg_iters_per_step"]))
+        out["cpu_baseline"] = cpu_baseline(it64, (2 * nstep + 1) * 2 * 15, acc, ell)
     out["job_s"] = time.perf_counter() - t_job
     print(json.dumps(out), flush=True)
 
