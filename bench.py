@@ -395,8 +395,8 @@ def cpu_baseline(iters_per_traj, force_poles, acceptance, ell):
     """The reference itself (its U(4) MPI build, oracle/_ref/susy_refbench_U4)
     on the host's cores, timed on the full 16^3 x 32 lattice: one fermion_op,
     3 iterations of the 15-shift congrad_multi and a 2-pole fermion_force;
-    a trajectory is priced at the GPU run's CG iteration count and force
-    poles.  Without that build (or without MPI), the C oracle on one core on
+    a trajectory is priced at the CG iteration count and force poles of the
+    plain algorithm (which the reference runs), from the FP64 leg.  Without that build (or without MPI), the C oracle on one core on
     an 8^3 x 16 sub-volume, scaled by volume."""
     import subprocess
     ranks = min(16, os.cpu_count() or 1)
@@ -418,7 +418,8 @@ def cpu_baseline(iters_per_traj, force_poles, acceptance, ell):
                                    f"iteration {r['cg_iteration_s']:.3f} s (3 timed), force "
                                    f"pole {r['force_pole_s']:.3f} s (2 timed); priced at "
                                    f"{iters_per_traj} CG iterations and {force_poles} force "
-                                   f"poles per trajectory (the GPU trajectory's counts), "
+                                   f"poles per trajectory (the plain algorithm's counts: the "
+                                   f"FP64 leg's start + 20 steps), "
                                    f"acceptance {acceptance:.2f} from the GPU run: est. "
                                    f"{t_traj:.0f} s per trajectory"),
                         "fermion_op_GBs": dslash_bytes(DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3], 16) / r["fermion_op_s"] / 1e9}
