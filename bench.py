@@ -38,11 +38,11 @@ Forms:
     multi-shift CG with FP64 reliable updates, FP64 forces, actions and dH,
     the learned surrogate force on the inner (gauge) MD level, and the
     rational action's large-shift poles (9..14, shifts >= 0.06) on a middle
-    MD level (one Omelyan step per half outer step, a multi-shift solve over
-    those six shifts only: 35-175 CG iterations) so that the outer level,
-    which keeps the small-shift poles, takes 3 steps per trajectory where the
-    plain forms take 5; `--warmup` untimed steps, then `--steps` timed steps,
-    trajectories completed and Metropolis-tested as they end.
+    MD level (two Omelyan steps per half outer step, each kick a multi-shift
+    solve over those six shifts only: ~80 CG iterations) so that the outer
+    level, which keeps the small-shift poles, takes 2 steps per trajectory
+    where the plain forms take 5; `--warmup` untimed steps, then `--steps`
+    timed steps, trajectories completed and Metropolis-tested as they end.
     `--split-first 0` runs the FP32 + surrogate form without the split at
     the plain forms' nstep.
   * FP32 (no surrogate) and plain FP64, nstep 5: the start of one trajectory
@@ -68,9 +68,10 @@ outcomes of the one or two trajectories a run of this length ends; the
 outcomes and dH are reported beside it).  The plain FP64 and FP32 forms
 integrate the same Hamiltonian with the same integrator (their dH on
 identical inputs agree to ~1e-4), so they share an acceptance rate; the
-split's 3 outer steps were chosen for an energy error no larger than the
-plain forms' at 5 (paired trajectories), so all forms are compared by
-trajectories per hour, and the ratios to the plain FP64 form are reported.
+split's 2 outer steps were chosen for an energy error like the plain
+integrator's at 5 (16^3 x 32, 6 trajectories each: <|dH|> 0.45 against
+0.48), so all forms are compared by trajectories per hour, and the ratios to
+the plain FP64 form are reported.
 The CPU baseline prices a trajectory of the reference (plain algorithm) at
 the FP64 leg's CG iterations.
 
@@ -105,7 +106,7 @@ SG_THETA_SPLIT = os.path.join(ROOT, "susy_amd", "configs", "surrogate_u4_split_t
 # headline form's pole split: poles SPLIT_FIRST..14 (shifts >= 0.06) on a
 # middle level of SPLIT_NMID Omelyan steps per half outer step, SPLIT_NSTEP
 # outer steps per trajectory of the same length
-SPLIT_FIRST, SPLIT_NMID, SPLIT_NSTEP = 9, 1, 3
+SPLIT_FIRST, SPLIT_NMID, SPLIT_NSTEP = 9, 2, 2
 
 
 def workload(traj_length, nstep, nstep_gauge):

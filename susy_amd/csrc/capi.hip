@@ -116,10 +116,8 @@ static void free_ctx(susy_ctx *c) {
                   c->d_force, c->f_links, c->f_uinv, c->f_pdet, c->d_aux, c->f_aux,
                   c->d_aux2, c->f_aux2,
                   c->d_tmp, c->f_tmp, c->d_boundary, c->d_part, c->d_scal, c->cg_ws,
-                  c->cg_t64, c->cg_state, c->f_in, c->f_out, c->d_faux};
+                  c->cg_t64, c->cg_state, c->f_in, c->f_out};
   for (void *p : ptrs)
-    if (p) (void)hipFree(p);
-  for (double2 *p : c->d_psol)
     if (p) (void)hipFree(p);
   for (double2 *p : c->tf)
     if (p) (void)hipFree(p);

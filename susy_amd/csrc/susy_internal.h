@@ -233,9 +233,6 @@ struct susy_ctx {
   double2 *d_dmu = nullptr;         // 1 matrix
   double2 *d_fmunu = nullptr;       // 10 matrices
   double2 *d_force = nullptr;       // 5 matrices (force accumulator)
-  std::vector<double2 *> d_psol;    // fermion force: D psi_n per pole (on first use)
-  double2 *d_faux = nullptr;        // fermion force: 12 complex per site per pole
-  int force_npole = 0;
   // pole-split integrator (susy_set_split): first middle-level pole, middle
   // steps per half outer step, middle-level weights (entries >= split_first)
   int split_first = 0, split_nmid = 1;
