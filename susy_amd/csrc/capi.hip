@@ -765,6 +765,10 @@ struct Traj {
   std::vector<std::vector<double2 *>> psim;
   std::vector<std::vector<double2 *>> psim_mid;   // middle-level solutions
   double amp_out[16] = {0};   // outer-level amplitudes
+  // middle level, fixed at susy_traj_begin: first pole, steps per half outer
+  // step, weights (entries >= mid_first)
+  int mid_first = 0, mid_steps = 1;
+  double mid_w[16] = {0};
   int mid_iters = 0;          // middle-level CG iterations
   double gnorm = 0, max_gf = 0, fnorm[2] = {0, 0}, max_ff[2] = {0, 0};
   int iters = 0;        // CG iterations so far
@@ -849,11 +853,12 @@ static int traj_cg(Traj &T, int root, int *iters) {
 }
 
 // Middle-level kick of the split form: for every root the multi-shift solve
-// over the large shifts only (shift4[j], j >= split_first; it converges in
-// few iterations) and the force of those poles with the middle weights.
+// over the large shifts only (shift4[j] from the first middle pole on; it
+// converges in few iterations) and the force of those poles with the middle
+// weights.
 static int mid_kick(Traj &T, double eps) {
   susy_ctx *c = T.c;
-  const int j0 = c->split_first, nm = c->p.Norder - j0;
+  const int j0 = T.mid_first, nm = c->p.Norder - j0;
   for (int n = 0; n < c->p.Nroot; n++) {
     double sr, fn;
     int it = cg_multi(c, T.gsrc[n], T.psim_mid[n].data(), nm, c->shift4 + j0, c->p.niter,
@@ -861,17 +866,17 @@ static int mid_kick(Traj &T, double eps) {
     if (it < 0) return it;
     T.iters += it;
     T.mid_iters += it;
-    RET(fermion_force_dev(c, eps, T.psim_mid[n].data(), nm, c->split_w + j0, &fn));
+    RET(fermion_force_dev(c, eps, T.psim_mid[n].data(), nm, T.mid_w + j0, &fn));
   }
   return 0;
 }
 
-// Middle level of the split form: split_nmid Omelyan steps advancing the
+// Middle level of the split form: mid_steps Omelyan steps advancing the
 // links by len, the middle kicks at the step points and the whole gauge
-// level (advancing len / (2 split_nmid)) between them.
+// level (advancing len / (2 mid_steps)) between them.
 static int mid_level(Traj &T, double len, double *norm) {
   susy_ctx *c = T.c;
-  const int n = c->split_nmid;
+  const int n = T.mid_steps;
   const double m = len / n, g = m / (2.0 * c->p.nstep_gauge);
   double tr, acc = 0;
   RET(mid_kick(T, m * OM_LAMBDA));
@@ -1018,8 +1023,13 @@ extern "C" int susy_traj_begin(susy_ctx *c, int form) {
   T.split = split;
   const int Nr = c->p.Nroot, No = c->p.Norder;
   const int nmid = split ? No - c->split_first : 0;
+  if (split) {
+    T.mid_first = c->split_first;
+    T.mid_steps = c->split_nmid;
+    for (int j = 0; j < No; j++) T.mid_w[j] = c->split_w[j];
+  }
   for (int j = 0; j < No; j++)
-    T.amp_out[j] = split && j >= c->split_first ? c->amp4[j] - c->split_w[j] : c->amp4[j];
+    T.amp_out[j] = split && j >= T.mid_first ? c->amp4[j] - T.mid_w[j] : c->amp4[j];
   if (!pure) {
     // per-trajectory fields are kept in the context across calls
     std::vector<double2 *> &pool = c->traj_pool;

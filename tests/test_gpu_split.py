@@ -75,3 +75,23 @@ def test_split_needs_its_setup_and_the_fermions():
     lat.set_split(10, 1)
     with pytest.raises(RuntimeError):
         lat.update(susy_amd.FORM_PUREGAUGE | susy_amd.FORM_SPLIT)
+
+
+def test_split_settings_are_fixed_for_the_open_trajectory():
+    """susy_set_split during a trajectory applies from the next one: the
+    open trajectory ends bitwise as it would have"""
+    def run(change):
+        lat = susy_amd.Lattice(2, **KW)
+        assert lat.load_config(config_path(2)) == 0
+        lat.set_split(FIRST, NMID)
+        lat.traj_begin(susy_amd.FORM_FP64 | susy_amd.FORM_SPLIT)
+        lat.traj_step()
+        if change:
+            lat.set_split(12, 1, np.ones(3))
+        for _ in range(KW["nstep"] - 1):
+            lat.traj_step()
+        st = lat.traj_end()
+        return st, lat.get_links()
+    st0, l0 = run(False)
+    st1, l1 = run(True)
+    assert np.array_equal(st0, st1) and np.array_equal(l0, l1)
