@@ -46,9 +46,9 @@ Forms:
     `--split-first 0` runs the FP32 + surrogate form without the split at
     the plain forms' nstep.
   * FP32 (no surrogate) and plain FP64, nstep 5: the start of one trajectory
-    and `--fp32-steps` / `--fp64-steps` timed steps from the same links and
-    seed; their trajectory time is start + 2 Nroot nstep x the mean step
-    time.
+    and `--fp32-steps` / `--fp64-steps` timed steps (default 2: both roots'
+    solves of the first half step) from the same links and seed; their
+    trajectory time is start + 2 Nroot nstep x the mean step time.
 The surrogate's parameters come from `python -m susy_amd.surrogate_train`:
 the library's own forward and backward kernels trained them (Adam) on
 (links, outer-level fermion force) pairs of the same form's RHMC on the
@@ -174,8 +174,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20,
                     help="timed steps (one root's solve + kick) of the FP32+surrogate form")
     ap.add_argument("--warmup", type=int, default=5, help="untimed outer steps before them")
-    ap.add_argument("--fp32-steps", type=int, default=1, help="timed steps, FP32 form")
-    ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps, plain FP64 form")
+    ap.add_argument("--fp32-steps", type=int, default=2,
+                    help="timed steps, FP32 form (2: both roots' solves of a half step)")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="timed steps, plain FP64 form (2: both roots' solves of a half step)")
     ap.add_argument("--traj-length", type=float, default=0.25)
     ap.add_argument("--nstep", type=int, default=5)
     ap.add_argument("--nstep-gauge", type=int, default=10)
@@ -380,7 +382,7 @@ def main():
     }
     if not args.no_cpu and trajs:
         # the reference runs the plain algorithm: a trajectory is priced at the
-        # FP64 leg's CG iterations (start + per_traj steps) and the plain
+        # FP64 leg's CG iterations (start + per_traj mean steps) and the plain
         # integrator's force poles
         it64 = int(round(f64["cg_iters_begin"] + per_traj * f64["cg_iters_per_step"]))
         out["cpu_baseline"] = cpu_baseline(it64, (2 * nstep + 1) * 2 * 15, acc, ell)
@@ -420,7 +422,7 @@ def cpu_baseline(iters_per_traj, force_poles, acceptance, ell):
                                    f"pole {r['force_pole_s']:.3f} s (2 timed); priced at "
                                    f"{iters_per_traj} CG iterations and {force_poles} force "
                                    f"poles per trajectory (the plain algorithm's counts: the "
-                                   f"FP64 leg's start + 20 steps), "
+                                   f"FP64 leg's start + 20 mean steps), "
                                    f"acceptance {acceptance:.2f} from the GPU run: est. "
                                    f"{t_traj:.0f} s per trajectory"),
                         "fermion_op_GBs": dslash_bytes(DIMS[0] * DIMS[1] * DIMS[2] * DIMS[3], 16) / r["fermion_op_s"] / 1e9}
